@@ -1,0 +1,671 @@
+/*
+ * ORACLE — test infrastructure only.  Nothing in the product links, loads or
+ * calls this file; tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg use it as the checker.
+ *
+ * A plain-C restatement of the reference's per-cell stiff chemistry
+ * integrator, driven by a mechanism table written by mkpp_amd.gen.plan:
+ *
+ *   rate laws ............ src/mkpp/lowering.py:199-313 (photolysis, Arrhenius,
+ *                          Troe/falloff with the B/C swap and N = 1, EP2, EP3,
+ *                          heterogeneous), third body AIR > M > air density
+ *                          (lowering.py:283, 296, 307)
+ *   F and J .............. lowering.py:315-372 mass-action scatter; J = dF/dC
+ *                          with placeholder species read as 1.0 after
+ *                          differentiation (src/mkpp/format_eqn.py:137-142)
+ *   W assembly ........... src/mkpp/templates/solver_variants/integrate.j2:103-107
+ *   factorize_plan ....... src/mkpp/templates/device_impl/supernodal_factorize.hpp.j2:10-31
+ *   solve_plan ........... src/mkpp/templates/device_impl/supernodal_solve.hpp.j2:10-42
+ *   integrate ............ integrate.j2:52-381 (stage sums, error norm over all
+ *                          N species, controller constants :71-82 and :324-379)
+ *   cell loop ............ src/mkpp_host/include/mkpp_host/dispatcher.hpp:224-254
+ *                          (per-cell T, air density and forcing row)
+ *
+ * Arithmetic follows the reference statement order; it is compiled with
+ * -ffp-contract=off like the reference's x86-64 host build.  The rate
+ * constant of a flux is evaluated with the same formula as the reference but
+ * not in SymPy's folded form, so results agree to rounding, not bit for bit.
+ *
+ * With hoist = 0 every F term and every J term re-evaluates its rate
+ * constant, as the generated code does (inline exp/pow in every expression);
+ * hoist = 1 evaluates each rate constant once per cell and call.  Both give
+ * identical numbers; only the cost differs.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAXTOK 32
+
+enum { LAW_PHOTO = 0, LAW_ARR, LAW_TROE, LAW_EP2, LAW_EP3, LAW_HET, LAW_CONST };
+
+typedef struct {
+    int ntok;
+    int kind[MAXTOK]; /* 0 number, 1 T, 2 + 3 - 4 * 5 / 6 ^ 7 neg 8 exp 9 log 10 sqrt */
+    double val[MAXTOK];
+} rpn_t;
+
+typedef struct {
+    int law, photo, npow, nnu, nparam, ndep;
+    rpn_t param[8];
+    int pow_s[4], pow_p[4];
+    int *nu_s;
+    double *nu_c;
+    int *dep;
+} group_t;
+
+typedef struct {
+    int n, nphoto, third_body, ngroups;
+    int *fixed, *unit;
+    group_t *g;
+    int *perm;
+    int nj, *ji, *jj, *jslot_of; /* jslot_of[i*n+j] physical -> slot or -1 */
+    int nw, *wi, *wj, *wfromj;
+    int nlu, nops, *op_out, *op_w, *op_b, *op_e, *op_d;
+    int nprod, *pl, *pr;
+    int *lo_off, nlower, *lo_c, *lo_s;
+    int *up_off, nupper, *up_c, *up_s;
+    int *diag;
+} mech_t;
+
+/* ------------------------------------------------------------------ */
+/* table reader                                                       */
+/* ------------------------------------------------------------------ */
+static int expect(FILE *f, const char *word) {
+    char buf[256];
+    if (fscanf(f, "%255s", buf) != 1 || strcmp(buf, word) != 0) {
+        fprintf(stderr, "oracle: expected '%s', got '%s'\n", word, buf);
+        return -1;
+    }
+    return 0;
+}
+
+static int *ialloc(int n) { return (int *)calloc(n > 0 ? n : 1, sizeof(int)); }
+static double *dalloc(int n) { return (double *)calloc(n > 0 ? n : 1, sizeof(double)); }
+
+static int read_rpn(FILE *f, rpn_t *r) {
+    char tok[128];
+    if (expect(f, "param")) return -1;
+    if (fscanf(f, "%d", &r->ntok) != 1 || r->ntok > MAXTOK) return -1;
+    for (int t = 0; t < r->ntok; ++t) {
+        if (fscanf(f, "%127s", tok) != 1) return -1;
+        r->val[t] = 0.0;
+        if (!strcmp(tok, "T")) r->kind[t] = 1;
+        else if (!strcmp(tok, "+")) r->kind[t] = 2;
+        else if (!strcmp(tok, "-")) r->kind[t] = 3;
+        else if (!strcmp(tok, "*")) r->kind[t] = 4;
+        else if (!strcmp(tok, "/")) r->kind[t] = 5;
+        else if (!strcmp(tok, "^")) r->kind[t] = 6;
+        else if (!strcmp(tok, "neg")) r->kind[t] = 7;
+        else if (!strcmp(tok, "exp")) r->kind[t] = 8;
+        else if (!strcmp(tok, "log")) r->kind[t] = 9;
+        else if (!strcmp(tok, "sqrt")) r->kind[t] = 10;
+        else {
+            r->kind[t] = 0;
+            r->val[t] = strtod(tok, NULL);
+        }
+    }
+    return 0;
+}
+
+static double eval_rpn(const rpn_t *r, double T) {
+    double st[MAXTOK];
+    int sp = 0;
+    for (int t = 0; t < r->ntok; ++t) {
+        double a, b;
+        switch (r->kind[t]) {
+        case 0: st[sp++] = r->val[t]; break;
+        case 1: st[sp++] = T; break;
+        case 2: b = st[--sp]; a = st[--sp]; st[sp++] = a + b; break;
+        case 3: b = st[--sp]; a = st[--sp]; st[sp++] = a - b; break;
+        case 4: b = st[--sp]; a = st[--sp]; st[sp++] = a * b; break;
+        case 5: b = st[--sp]; a = st[--sp]; st[sp++] = a / b; break;
+        case 6: b = st[--sp]; a = st[--sp]; st[sp++] = pow(a, b); break;
+        case 7: st[sp - 1] = -st[sp - 1]; break;
+        case 8: st[sp - 1] = exp(st[sp - 1]); break;
+        case 9: st[sp - 1] = log(st[sp - 1]); break;
+        case 10: st[sp - 1] = sqrt(st[sp - 1]); break;
+        }
+    }
+    return st[0];
+}
+
+
+void oracle_free(mech_t *m);
+
+mech_t *oracle_load(const char *path) {
+    FILE *f = fopen(path, "r");
+    if (!f) return NULL;
+    mech_t *m = (mech_t *)calloc(1, sizeof(mech_t));
+    char name[256];
+    int ok = 1, dummy;
+    ok &= !expect(f, "mechanism") && fscanf(f, "%255s", name) == 1;
+    ok &= !expect(f, "species") && fscanf(f, "%d", &m->n) == 1;
+    if (!ok) goto fail;
+    m->fixed = ialloc(m->n);
+    m->unit = ialloc(m->n);
+    for (int i = 0; i < m->n; ++i) {
+        ok &= !expect(f, "sp") && fscanf(f, "%d %255s %d %d", &dummy, name, &m->fixed[i], &m->unit[i]) == 4;
+    }
+    ok &= !expect(f, "third_body") && fscanf(f, "%d", &m->third_body) == 1;
+    ok &= !expect(f, "nphoto") && fscanf(f, "%d", &m->nphoto) == 1;
+    ok &= !expect(f, "groups") && fscanf(f, "%d", &m->ngroups) == 1;
+    if (!ok) goto fail;
+    m->g = (group_t *)calloc(m->ngroups > 0 ? m->ngroups : 1, sizeof(group_t));
+    for (int q = 0; q < m->ngroups && ok; ++q) {
+        group_t *g = &m->g[q];
+        ok &= !expect(f, "group") && fscanf(f, "%d %d %d %d %d", &g->law, &g->photo, &g->npow, &g->nnu, &g->nparam) == 5;
+        if (!ok || g->npow > 4) goto fail;
+        for (int k = 0; k < g->nparam; ++k) ok &= !read_rpn(f, &g->param[k]);
+        ok &= !expect(f, "powers");
+        for (int k = 0; k < g->npow; ++k) ok &= fscanf(f, "%d %d", &g->pow_s[k], &g->pow_p[k]) == 2;
+        ok &= !expect(f, "nu");
+        g->nu_s = ialloc(g->nnu);
+        g->nu_c = dalloc(g->nnu);
+        for (int k = 0; k < g->nnu; ++k) ok &= fscanf(f, "%d %lf", &g->nu_s[k], &g->nu_c[k]) == 2;
+        ok &= !expect(f, "deps") && fscanf(f, "%d", &g->ndep) == 1;
+        g->dep = ialloc(g->ndep);
+        for (int k = 0; k < g->ndep; ++k) ok &= fscanf(f, "%d", &g->dep[k]) == 1;
+    }
+    m->perm = ialloc(m->n);
+    ok &= !expect(f, "perm");
+    for (int i = 0; i < m->n; ++i) ok &= fscanf(f, "%d", &m->perm[i]) == 1;
+    ok &= !expect(f, "jslots") && fscanf(f, "%d", &m->nj) == 1;
+    if (!ok) goto fail;
+    m->ji = ialloc(m->nj);
+    m->jj = ialloc(m->nj);
+    m->jslot_of = ialloc(m->n * m->n);
+    for (int i = 0; i < m->n * m->n; ++i) m->jslot_of[i] = -1;
+    for (int s = 0; s < m->nj; ++s) {
+        ok &= fscanf(f, "%d %d", &m->ji[s], &m->jj[s]) == 2;
+        m->jslot_of[m->ji[s] * m->n + m->jj[s]] = s;
+    }
+    ok &= !expect(f, "wslots") && fscanf(f, "%d", &m->nw) == 1;
+    if (!ok) goto fail;
+    m->wi = ialloc(m->nw);
+    m->wj = ialloc(m->nw);
+    m->wfromj = ialloc(m->nw);
+    for (int s = 0; s < m->nw; ++s) ok &= fscanf(f, "%d %d %d", &m->wi[s], &m->wj[s], &m->wfromj[s]) == 3;
+    ok &= !expect(f, "luslots") && fscanf(f, "%d", &m->nlu) == 1;
+    ok &= !expect(f, "ops") && fscanf(f, "%d", &m->nops) == 1;
+    if (!ok) goto fail;
+    m->op_out = ialloc(m->nops);
+    m->op_w = ialloc(m->nops);
+    m->op_b = ialloc(m->nops);
+    m->op_e = ialloc(m->nops);
+    m->op_d = ialloc(m->nops);
+    for (int o = 0; o < m->nops; ++o)
+        ok &= fscanf(f, "%d %d %d %d %d", &m->op_out[o], &m->op_w[o], &m->op_b[o], &m->op_e[o], &m->op_d[o]) == 5;
+    ok &= !expect(f, "products") && fscanf(f, "%d", &m->nprod) == 1;
+    if (!ok) goto fail;
+    m->pl = ialloc(m->nprod);
+    m->pr = ialloc(m->nprod);
+    for (int p = 0; p < m->nprod; ++p) ok &= fscanf(f, "%d %d", &m->pl[p], &m->pr[p]) == 2;
+    m->lo_off = ialloc(m->n + 1);
+    m->up_off = ialloc(m->n + 1);
+    ok &= !expect(f, "lower_offsets");
+    for (int i = 0; i <= m->n; ++i) ok &= fscanf(f, "%d", &m->lo_off[i]) == 1;
+    ok &= !expect(f, "lower") && fscanf(f, "%d", &m->nlower) == 1;
+    if (!ok) goto fail;
+    m->lo_c = ialloc(m->nlower);
+    m->lo_s = ialloc(m->nlower);
+    for (int e = 0; e < m->nlower; ++e) ok &= fscanf(f, "%d %d", &m->lo_c[e], &m->lo_s[e]) == 2;
+    ok &= !expect(f, "upper_offsets");
+    for (int i = 0; i <= m->n; ++i) ok &= fscanf(f, "%d", &m->up_off[i]) == 1;
+    ok &= !expect(f, "upper") && fscanf(f, "%d", &m->nupper) == 1;
+    if (!ok) goto fail;
+    m->up_c = ialloc(m->nupper);
+    m->up_s = ialloc(m->nupper);
+    for (int e = 0; e < m->nupper; ++e) ok &= fscanf(f, "%d %d", &m->up_c[e], &m->up_s[e]) == 2;
+    m->diag = ialloc(m->n);
+    ok &= !expect(f, "diag");
+    for (int i = 0; i < m->n; ++i) ok &= fscanf(f, "%d", &m->diag[i]) == 1;
+    ok &= !expect(f, "end");
+    if (!ok) goto fail;
+    fclose(f);
+    return m;
+fail:
+    fclose(f);
+    oracle_free(m);
+    return NULL;
+}
+
+void oracle_free(mech_t *m) {
+    if (!m) return;
+    for (int q = 0; q < m->ngroups && m->g; ++q) {
+        free(m->g[q].nu_s);
+        free(m->g[q].nu_c);
+        free(m->g[q].dep);
+    }
+    free(m->g);
+    free(m->fixed); free(m->unit); free(m->perm);
+    free(m->ji); free(m->jj); free(m->jslot_of);
+    free(m->wi); free(m->wj); free(m->wfromj);
+    free(m->op_out); free(m->op_w); free(m->op_b); free(m->op_e); free(m->op_d);
+    free(m->pl); free(m->pr);
+    free(m->lo_off); free(m->lo_c); free(m->lo_s);
+    free(m->up_off); free(m->up_c); free(m->up_s);
+    free(m->diag);
+    free(m);
+}
+
+int oracle_n(const mech_t *m) { return m->n; }
+int oracle_nphoto(const mech_t *m) { return m->nphoto; }
+int oracle_nj(const mech_t *m) { return m->nj; }
+int oracle_nw(const mech_t *m) { return m->nw; }
+int oracle_nlu(const mech_t *m) { return m->nlu; }
+int oracle_ngroups(const mech_t *m) { return m->ngroups; }
+
+/* ------------------------------------------------------------------ */
+/* rate constants (lowering.py:199-313)                               */
+/* ------------------------------------------------------------------ */
+static double third_body_value(const mech_t *m, const double *y, double air_density) {
+    return m->third_body >= 0 ? y[m->third_body] : air_density;
+}
+
+/* k and dk/d[M] of one flux group. */
+static void group_rate(const mech_t *m, const group_t *g, double T, double M, const double *jv, double *k, double *dk) {
+    const rpn_t *p = g->param;
+    *dk = 0.0;
+    switch (g->law) {
+    case LAW_PHOTO: *k = jv[g->photo]; break;
+    case LAW_ARR: {
+        double A = eval_rpn(&p[0], T), B = eval_rpn(&p[1], T), C = eval_rpn(&p[2], T);
+        *k = A * pow(T / 300.0, B) * exp(C / T);
+        break;
+    }
+    case LAW_TROE: {
+        double k0 = eval_rpn(&p[0], T) * exp(eval_rpn(&p[2], T) / T) * pow(T / 300.0, eval_rpn(&p[1], T));
+        double ki = eval_rpn(&p[3], T) * exp(eval_rpn(&p[5], T) / T) * pow(T / 300.0, eval_rpn(&p[4], T));
+        double fc = eval_rpn(&p[6], T);
+        double K0 = k0 * M;
+        double r = K0 / ki;
+        double lg = log10(r);
+        double ex = 1.0 / (1.0 + lg * lg);
+        double F = pow(fc, ex);
+        *k = K0 / (1.0 + r) * F;
+        double dr = k0 / ki;
+        double dex = -ex * ex * 2.0 * lg / (r * log(10.0)) * dr;
+        double dF = F * log(fc) * dex;
+        *dk = (k0 / (1.0 + r) - K0 * dr / ((1.0 + r) * (1.0 + r))) * F + K0 / (1.0 + r) * dF;
+        break;
+    }
+    case LAW_EP2: {
+        double K0 = eval_rpn(&p[0], T) * exp(eval_rpn(&p[1], T) / T);
+        double K2 = eval_rpn(&p[2], T) * exp(eval_rpn(&p[3], T) / T);
+        double k3 = eval_rpn(&p[4], T) * exp(eval_rpn(&p[5], T) / T);
+        double K3 = k3 * M;
+        *k = K0 + K3 / (1.0 + K3 / K2);
+        *dk = k3 / ((1.0 + K3 / K2) * (1.0 + K3 / K2));
+        break;
+    }
+    case LAW_EP3: {
+        double K1 = eval_rpn(&p[0], T) * exp(eval_rpn(&p[1], T) / T);
+        double K2 = eval_rpn(&p[2], T) * exp(eval_rpn(&p[3], T) / T);
+        *k = K1 + K2 * M;
+        *dk = K2;
+        break;
+    }
+    case LAW_HET: *k = 0.25 * eval_rpn(&p[0], T); break;
+    default: *k = eval_rpn(&p[0], T); break;
+    }
+}
+
+typedef struct {
+    double T, air_density;
+    const double *jv;
+    int hoist;
+    double *k, *dk; /* per group, valid when hoist */
+} env_t;
+
+static void env_rates(const mech_t *m, env_t *e, const double *y) {
+    double M = third_body_value(m, y, e->air_density);
+    for (int q = 0; q < m->ngroups; ++q) group_rate(m, &m->g[q], e->T, M, e->jv, &e->k[q], &e->dk[q]);
+}
+
+static inline double yval(const mech_t *m, const double *y, int s) { return m->unit[s] ? 1.0 : y[s]; }
+
+static inline double ipow(double x, int p) {
+    double r = x;
+    for (int i = 1; i < p; ++i) r *= x;
+    return p == 0 ? 1.0 : r;
+}
+
+/* F in physical order. */
+static void eval_rates(const mech_t *m, env_t *e, const double *y, double *f) {
+    double M = third_body_value(m, y, e->air_density);
+    for (int i = 0; i < m->n; ++i) f[i] = 0.0;
+    for (int q = 0; q < m->ngroups; ++q) {
+        const group_t *g = &m->g[q];
+        double mono = 1.0;
+        for (int a = 0; a < g->npow; ++a) mono *= ipow(yval(m, y, g->pow_s[a]), g->pow_p[a]);
+        for (int t = 0; t < g->nnu; ++t) {
+            double k, dk;
+            if (e->hoist) k = e->k[q];
+            else group_rate(m, g, e->T, M, e->jv, &k, &dk);
+            f[g->nu_s[t]] += g->nu_c[t] * (k * mono);
+        }
+    }
+}
+
+/* J per slot (physical row-major slot order). */
+static void eval_jacobian(const mech_t *m, env_t *e, const double *y, double *jv) {
+    double M = third_body_value(m, y, e->air_density);
+    for (int s = 0; s < m->nj; ++s) jv[s] = 0.0;
+    for (int q = 0; q < m->ngroups; ++q) {
+        const group_t *g = &m->g[q];
+        for (int d = 0; d < g->ndep; ++d) {
+            int j = g->dep[d];
+            for (int t = 0; t < g->nnu; ++t) {
+                double k, dk;
+                if (e->hoist) { k = e->k[q]; dk = e->dk[q]; }
+                else group_rate(m, g, e->T, M, e->jv, &k, &dk);
+                double part = 0.0;
+                for (int a = 0; a < g->npow; ++a) {
+                    if (g->pow_s[a] != j) continue;
+                    double term = k * (double)g->pow_p[a] * ipow(yval(m, y, j), g->pow_p[a] - 1);
+                    for (int b = 0; b < g->npow; ++b)
+                        if (b != a) term *= ipow(yval(m, y, g->pow_s[b]), g->pow_p[b]);
+                    part += term;
+                }
+                if ((g->law == LAW_TROE || g->law == LAW_EP2 || g->law == LAW_EP3) && j == m->third_body) {
+                    double term = dk;
+                    for (int b = 0; b < g->npow; ++b) term *= ipow(yval(m, y, g->pow_s[b]), g->pow_p[b]);
+                    part += term;
+                }
+                int slot = m->jslot_of[g->nu_s[t] * m->n + j];
+                jv[slot] += g->nu_c[t] * part;
+            }
+        }
+    }
+}
+
+/* factorize_plan (supernodal_factorize.hpp.j2:10-31). */
+void oracle_factorize(const mech_t *m, const double *w, double *lu) {
+    for (int o = 0; o < m->nops; ++o) {
+        double v = m->op_w[o] >= 0 ? w[m->op_w[o]] : 0.0;
+        for (int p = m->op_b[o]; p < m->op_e[o]; ++p) v -= lu[m->pl[p]] * lu[m->pr[p]];
+        lu[m->op_out[o]] = m->op_d[o] >= 0 ? v / lu[m->op_d[o]] : v;
+    }
+}
+
+/* solve_plan (supernodal_solve.hpp.j2:10-42); logical order in and out. */
+void oracle_solve(const mech_t *m, const double *lu, const double *rhs, double *x, double *work) {
+    int n = m->n;
+    for (int r = 0; r < n; ++r) {
+        work[r] = rhs[r];
+        for (int e = m->lo_off[r]; e < m->lo_off[r + 1]; ++e) work[r] -= lu[m->lo_s[e]] * work[m->lo_c[e]];
+    }
+    for (int r = n - 1; r >= 0; --r) {
+        x[r] = work[r];
+        for (int e = m->up_off[r]; e < m->up_off[r + 1]; ++e) x[r] -= lu[m->up_s[e]] * x[m->up_c[e]];
+        x[r] /= lu[m->diag[r]];
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* tableaux (Sandu et al. 1997 / KPP; src/mkpp/rosenbrock.py:27-133)  */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int s;
+    double a[15], c[15], mm[6], ee[6], g0, elo;
+    int newf[6];
+} tab_t;
+
+static void get_tableau(int method, tab_t *t) {
+    memset(t, 0, sizeof(*t));
+    switch (method) {
+    case 0: { /* ros2 */
+        static const double a[] = {1.7071067811865475244}, c[] = {-1.1715728752538099024};
+        static const double mm[] = {0.87867965644035742774, 0.29289321881345247560}, ee[] = {0.29289321881345247560, 0.29289321881345247560};
+        t->s = 2; memcpy(t->a, a, sizeof a); memcpy(t->c, c, sizeof c); memcpy(t->mm, mm, sizeof mm); memcpy(t->ee, ee, sizeof ee);
+        t->g0 = 1.7071067811865475244; t->elo = 2.0; t->newf[0] = t->newf[1] = 1;
+        break;
+    }
+    case 1: { /* ros3 */
+        static const double a[] = {1.0, 1.0, 0.0}, c[] = {-1.0156171083877702092, 4.0759956452537699825, 9.2076794298330791242};
+        static const double mm[] = {1.0, 6.1697947043828245593, -0.42772256543218573326}, ee[] = {0.5, -2.9079558716805469822, 0.22354069897811569627};
+        t->s = 3; memcpy(t->a, a, sizeof a); memcpy(t->c, c, sizeof c); memcpy(t->mm, mm, sizeof mm); memcpy(t->ee, ee, sizeof ee);
+        t->g0 = 0.43586652150845899942; t->elo = 3.0; t->newf[0] = t->newf[1] = 1; t->newf[2] = 0;
+        break;
+    }
+    case 2: { /* ros4 */
+        static const double a[] = {2.0, 1.867943637803922, 0.2344449711399156, 1.867943637803922, 0.2344449711399156, 0.0};
+        static const double c[] = {-7.137615036412310, 2.580708087951457, 0.6515950076447975, -2.137148994382534, -0.3214669691237626, -0.6949742501781779};
+        static const double mm[] = {2.255570073418735, 0.2870493262186792, 0.4353179431840180, 1.093502252409163};
+        static const double ee[] = {-0.2815431932141155, -0.07276199124938920, -0.1082196201495311, -1.093502252409163};
+        t->s = 4; memcpy(t->a, a, sizeof a); memcpy(t->c, c, sizeof c); memcpy(t->mm, mm, sizeof mm); memcpy(t->ee, ee, sizeof ee);
+        t->g0 = 0.5728200000000000; t->elo = 4.0; t->newf[0] = t->newf[1] = t->newf[2] = 1; t->newf[3] = 0;
+        break;
+    }
+    case 3: { /* rodas3 */
+        static const double a[] = {0.0, 2.0, 0.0, 2.0, 0.0, 1.0}, c[] = {4.0, 1.0, -1.0, 1.0, -1.0, -8.0 / 3.0};
+        static const double mm[] = {2.0, 0.0, 1.0, 1.0}, ee[] = {0.0, 0.0, 0.0, 1.0};
+        t->s = 4; memcpy(t->a, a, sizeof a); memcpy(t->c, c, sizeof c); memcpy(t->mm, mm, sizeof mm); memcpy(t->ee, ee, sizeof ee);
+        t->g0 = 0.5; t->elo = 3.0; t->newf[0] = 1; t->newf[1] = 0; t->newf[2] = t->newf[3] = 1;
+        break;
+    }
+    default: { /* rodas4 */
+        static const double a[] = {1.544000000000000, 0.9466785280815826, 0.2557011698983284, 3.314825187068521, 2.896124015972201,
+                                   0.9986419139977817, 1.221224509226641, 6.019134481288629, 12.53708332932087, -0.6878860361058950,
+                                   1.221224509226641, 6.019134481288629, 12.53708332932087, -0.6878860361058950, 1.0};
+        static const double c[] = {-5.668800000000000, -2.430093356833875, -0.2063599157091915, -0.1073529058151375, -9.594562251023355,
+                                   -20.47028614809616, 7.496443313967647, -10.24680431464352, -33.99990352819905, 11.70890893206160,
+                                   8.083246795921522, -7.981132988064893, -31.52159432874371, 16.31930543123136, -6.058818238834054};
+        static const double mm[] = {1.221224509226641, 6.019134481288629, 12.53708332932087, -0.6878860361058950, 1.0, 1.0};
+        static const double ee[] = {0.0, 0.0, 0.0, 0.0, 0.0, 1.0};
+        t->s = 6; memcpy(t->a, a, sizeof a); memcpy(t->c, c, sizeof c); memcpy(t->mm, mm, sizeof mm); memcpy(t->ee, ee, sizeof ee);
+        t->g0 = 0.25; t->elo = 4.0; for (int i = 0; i < 6; ++i) t->newf[i] = 1;
+        break;
+    }
+    }
+}
+
+/* Stage sum in the template's textual form: base + c1*K1 + c2*K2 ... with a
+ * coefficient of exactly 1.0 written as a bare term (integrate.j2:147-174).
+ * scale_by_dt: each term is (c / dt) * K (integrate.j2:225-249). */
+static inline double stage_sum(double base, const double *coef, int nterm, double *const *K, int i, double dt, int scale_by_dt) {
+    double v = base;
+    for (int j = 0; j < nterm; ++j) {
+        double c = coef[j];
+        if (c == 0.0) continue;
+        if (scale_by_dt) v = v + (c / dt) * K[j][i];
+        else if (c == 1.0) v = v + K[j][i];
+        else v = v + c * K[j][i];
+    }
+    return v;
+}
+
+/* Sum of M or E terms alone (the "m_expr"/"e_expr" of integrate.j2:267-300). */
+static inline double weight_sum(const double *w, int s, double *const *K, int i, int *any) {
+    double v = 0.0;
+    int first = 1;
+    for (int j = 0; j < s; ++j) {
+        if (w[j] == 0.0) continue;
+        double term = (w[j] == 1.0) ? K[j][i] : w[j] * K[j][i];
+        if (first) { v = term; first = 0; }
+        else v = v + term;
+    }
+    *any = !first;
+    return v;
+}
+
+typedef struct {
+    double *S, *Y, *F, *rhs, *K[6], *J, *W, *LU, *work, *phys, *fphys, *k, *dk;
+} scratch_t;
+
+static void scratch_alloc(const mech_t *m, scratch_t *s) {
+    int n = m->n;
+    s->S = dalloc(n); s->Y = dalloc(n); s->F = dalloc(n); s->rhs = dalloc(n);
+    for (int i = 0; i < 6; ++i) s->K[i] = dalloc(n);
+    s->J = dalloc(m->nj); s->W = dalloc(m->nw); s->LU = dalloc(m->nlu); s->work = dalloc(n);
+    s->phys = dalloc(n); s->fphys = dalloc(n); s->k = dalloc(m->ngroups); s->dk = dalloc(m->ngroups);
+}
+
+static void scratch_free(scratch_t *s) {
+    free(s->S); free(s->Y); free(s->F); free(s->rhs);
+    for (int i = 0; i < 6; ++i) free(s->K[i]);
+    free(s->J); free(s->W); free(s->LU); free(s->work); free(s->phys); free(s->fphys); free(s->k); free(s->dk);
+}
+
+/* One cell's integrate(dt_total): state in physical order, updated in place.
+ * Returns the number of attempted steps (accepted in *acc). */
+static long integrate_cell(const mech_t *m, const tab_t *tb, double *state, env_t *e, double dt_total, double seed,
+                           const double *atol, const double *rtol, scratch_t *s, long *acc, long max_attempts) {
+    const int n = m->n;
+    const double g = tb->g0, safety = 0.9, max_growth = 6.0, min_shrink = 0.2, rej_dec = 0.1;
+    const double h_min = dt_total * 1.0e-15;
+    double t = 0.0, dt = dt_total * seed;
+    int reject_last = 0, reject_more = 0;
+    long attempts = 0, accepted = 0;
+    if (e->hoist) env_rates(m, e, state); /* T and fixed species do not change within the call */
+    while (t < dt_total) {
+        if (max_attempts > 0 && attempts >= max_attempts) break;
+        ++attempts;
+        dt = fmin(dt, dt_total - t);
+        const double inv_g_dt = 1.0 / (g * dt);
+        for (int i = 0; i < n; ++i) s->S[i] = state[m->perm[i]];
+        eval_jacobian(m, e, state, s->J);
+        for (int w = 0; w < m->nw; ++w) {
+            double base = (m->wi[w] == m->wj[w]) ? inv_g_dt : 0.0;
+            s->W[w] = m->wfromj[w] >= 0 ? base - s->J[m->wfromj[w]] : base;
+        }
+        oracle_factorize(m, s->W, s->LU);
+        for (int st = 0; st < tb->s; ++st) {
+            int base_a = st * (st - 1) / 2;
+            if (st > 0) {
+                for (int i = 0; i < n; ++i) s->Y[i] = stage_sum(s->S[i], &tb->a[base_a], st, s->K, i, dt, 0);
+            }
+            if (tb->newf[st]) {
+                if (st == 0) {
+                    eval_rates(m, e, state, s->fphys);
+                } else {
+                    for (int i = 0; i < n; ++i) s->phys[m->perm[i]] = s->Y[i];
+                    eval_rates(m, e, s->phys, s->fphys);
+                }
+                for (int i = 0; i < n; ++i) s->F[i] = s->fphys[m->perm[i]];
+            }
+            if (st == 0) {
+                for (int i = 0; i < n; ++i) s->rhs[i] = s->F[i];
+            } else {
+                for (int i = 0; i < n; ++i) s->rhs[i] = stage_sum(s->F[i], &tb->c[base_a], st, s->K, i, dt, 1);
+            }
+            oracle_solve(m, s->LU, s->rhs, s->K[st], s->work);
+        }
+        double err_sq = 0.0;
+        for (int i = 0; i < n; ++i) {
+            int pi = m->perm[i], any;
+            /* Ynew_i = S_i + term1 + term2 ... evaluated left to right (integrate.j2:305-310) */
+            double ynew = s->S[i];
+            for (int j = 0; j < tb->s; ++j) {
+                if (tb->mm[j] == 0.0) continue;
+                ynew = ynew + ((tb->mm[j] == 1.0) ? s->K[j][i] : tb->mm[j] * s->K[j][i]);
+            }
+            double ymax = fmax(fabs(state[pi]), fabs(ynew));
+            double sci = atol[pi] + rtol[pi] * ymax;
+            double yerr = weight_sum(tb->ee, tb->s, s->K, i, &any);
+            err_sq += (yerr / sci) * (yerr / sci);
+        }
+        double err = sqrt(err_sq / n);
+        err = fmax(err, 1.0e-10);
+        double factor;
+        if (tb->elo == 2.0) factor = safety / sqrt(err);
+        else if (tb->elo == 3.0) factor = safety / cbrt(err);
+        else factor = safety * pow(err, -(1.0 / tb->elo));
+        factor = fmax(min_shrink, fmin(factor, max_growth));
+        if (err <= 1.0) {
+            for (int i = 0; i < n; ++i) {
+                int any;
+                double msum = weight_sum(tb->mm, tb->s, s->K, i, &any);
+                state[m->perm[i]] += any ? msum : 0.0;
+            }
+            t += dt;
+            ++accepted;
+            double next_dt = fmax(h_min, fmin(dt * factor, dt_total));
+            if (reject_last) next_dt = fmin(next_dt, dt);
+            reject_last = 0;
+            reject_more = 0;
+            dt = next_dt;
+        } else {
+            dt *= reject_more ? rej_dec : factor;
+            reject_more = reject_last;
+            reject_last = 1;
+        }
+    }
+    *acc = accepted;
+    return attempts;
+}
+
+/* ------------------------------------------------------------------ */
+/* public entry points (ctypes)                                       */
+/* ------------------------------------------------------------------ */
+
+/* F at one state, physical order. */
+void oracle_rates(const mech_t *m, double T, double air_density, const double *y, const double *jv, double *f) {
+    env_t e = {T, air_density, jv, 0, NULL, NULL};
+    eval_rates(m, &e, y, f);
+}
+
+/* J values per J slot. */
+void oracle_jacobian(const mech_t *m, double T, double air_density, const double *y, const double *jv, double *jslots) {
+    env_t e = {T, air_density, jv, 0, NULL, NULL};
+    eval_jacobian(m, &e, y, jslots);
+}
+
+/* Rate constant and dk/d[M] per flux group. */
+void oracle_rate_constants(const mech_t *m, double T, double air_density, const double *y, const double *jv, double *k, double *dk) {
+    env_t e = {T, air_density, jv, 1, k, dk};
+    env_rates(m, &e, y);
+}
+
+/*
+ * Integrate num_cells cells, each `steps` times over dt_total, in the
+ * C-ABI's LayoutLeft layout: conc[cell + num_cells*species],
+ * jvals[cell + num_cells*slot]; temp and air_density per cell.
+ * accepted/attempted (may be NULL) receive per-cell totals.
+ * Returns the number of cells that hit max_attempts (0 = none).
+ */
+int oracle_integrate(const mech_t *m, int method, long num_cells, double *conc, const double *temp, const double *air_density,
+                     const double *jvals, double dt_total, int steps, double seed, const double *atol, const double *rtol,
+                     long *accepted, long *attempted, int hoist, long max_attempts, int nthreads) {
+    tab_t tb;
+    get_tableau(method, &tb);
+    int stalled = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel reduction(+ : stalled)
+#endif
+    {
+        scratch_t s;
+        scratch_alloc(m, &s);
+        double *state = dalloc(m->n);
+        double *jv = dalloc(m->nphoto);
+#ifdef _OPENMP
+#pragma omp for schedule(static, 64)
+#endif
+        for (long c = 0; c < num_cells; ++c) {
+            for (int i = 0; i < m->n; ++i) state[i] = conc[c + num_cells * i];
+            for (int p = 0; p < m->nphoto; ++p) jv[p] = jvals ? jvals[c + num_cells * p] : 0.0;
+            env_t e = {temp ? temp[c] : 300.0, air_density ? air_density[c] : 2.4476e19, jv, hoist, s.k, s.dk};
+            long acc_total = 0, att_total = 0;
+            for (int st = 0; st < steps; ++st) {
+                long acc = 0;
+                long att = integrate_cell(m, &tb, state, &e, dt_total, seed, atol, rtol, &s, &acc, max_attempts);
+                if (max_attempts > 0 && att >= max_attempts) stalled += 1;
+                acc_total += acc;
+                att_total += att;
+            }
+            for (int i = 0; i < m->n; ++i) conc[c + num_cells * i] = state[i];
+            if (accepted) accepted[c] = acc_total;
+            if (attempted) attempted[c] = att_total;
+        }
+        free(state);
+        free(jv);
+        scratch_free(&s);
+    }
+    return stalled;
+}

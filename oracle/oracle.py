@@ -1,0 +1,141 @@
+"""ctypes binding of the C oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, as the checker.  The product (mkpp_amd) never does.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libmkpp_oracle.so")
+TABLES = os.path.join(HERE, "tables")
+METHODS = {"ros2": 0, "ros3": 1, "ros4": 2, "rodas3": 3, "rodas4": 4}
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        vp, dp, lp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_long)
+        L.oracle_load.restype = vp
+        L.oracle_load.argtypes = [C.c_char_p]
+        L.oracle_free.argtypes = [vp]
+        for nm in ("oracle_n", "oracle_nphoto", "oracle_nj", "oracle_nw", "oracle_nlu", "oracle_ngroups"):
+            getattr(L, nm).argtypes = [vp]
+            getattr(L, nm).restype = C.c_int
+        L.oracle_rates.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp]
+        L.oracle_jacobian.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp]
+        L.oracle_rate_constants.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp, dp]
+        L.oracle_factorize.argtypes = [vp, dp, dp]
+        L.oracle_solve.argtypes = [vp, dp, dp, dp, dp]
+        L.oracle_integrate.restype = C.c_int
+        L.oracle_integrate.argtypes = [vp, C.c_int, C.c_long, dp, dp, dp, dp, C.c_double, C.c_int, C.c_double,
+                                       dp, dp, lp, lp, C.c_int, C.c_long, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _l(a):
+    return a.ctypes.data_as(C.POINTER(C.c_long)) if a is not None else None
+
+
+class Oracle:
+    """One mechanism's CPU reference integrator."""
+
+    def __init__(self, mech: str, path: str | None = None):
+        self.name = mech
+        path = path or os.path.join(TABLES, f"{mech}.tab")
+        self._h = lib().oracle_load(path.encode())
+        if not self._h:
+            raise RuntimeError(f"oracle: cannot load {path}")
+        L = lib()
+        self.n = L.oracle_n(self._h)
+        self.nphoto = L.oracle_nphoto(self._h)
+        self.nj = L.oracle_nj(self._h)
+        self.nw = L.oracle_nw(self._h)
+        self.nlu = L.oracle_nlu(self._h)
+        self.ngroups = L.oracle_ngroups(self._h)
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().oracle_free(self._h)
+        except Exception:
+            pass
+
+    def rates(self, y, jvals=(), T=300.0, air_density=2.4476e19):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        jv = np.ascontiguousarray(np.asarray(jvals, dtype=np.float64).reshape(-1) if len(jvals) else np.zeros(1))
+        f = np.zeros(self.n)
+        lib().oracle_rates(self._h, T, air_density, _d(y), _d(jv), _d(f))
+        return f
+
+    def jacobian_slots(self, y, jvals=(), T=300.0, air_density=2.4476e19):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        jv = np.ascontiguousarray(np.asarray(jvals, dtype=np.float64).reshape(-1) if len(jvals) else np.zeros(1))
+        out = np.zeros(self.nj)
+        lib().oracle_jacobian(self._h, T, air_density, _d(y), _d(jv), _d(out))
+        return out
+
+    def rate_constants(self, y, jvals=(), T=300.0, air_density=2.4476e19):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        jv = np.ascontiguousarray(np.asarray(jvals, dtype=np.float64).reshape(-1) if len(jvals) else np.zeros(1))
+        k = np.zeros(max(self.ngroups, 1))
+        dk = np.zeros(max(self.ngroups, 1))
+        lib().oracle_rate_constants(self._h, T, air_density, _d(y), _d(jv), _d(k), _d(dk))
+        return k[: self.ngroups], dk[: self.ngroups]
+
+    def factorize(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        lu = np.zeros(self.nlu)
+        lib().oracle_factorize(self._h, _d(w), _d(lu))
+        return lu
+
+    def solve(self, lu, rhs):
+        lu = np.ascontiguousarray(lu, dtype=np.float64)
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.zeros(self.n)
+        work = np.zeros(self.n)
+        lib().oracle_solve(self._h, _d(lu), _d(rhs), _d(x), _d(work))
+        return x
+
+    def integrate(self, conc, dt, steps=1, method="rodas3", temp=None, air_density=None, jvals=None,
+                  atol=None, rtol=None, seed=1.0e-6, hoist=True, max_attempts=0, threads=0):
+        """Integrate cells in place.  conc: (num_species, num_cells) float64,
+        i.e. the LayoutLeft buffer conc[cell + num_cells*species].
+        Returns (accepted, attempted, stalled_cells)."""
+        conc = np.asarray(conc)
+        assert conc.dtype == np.float64 and conc.flags.c_contiguous and conc.shape[0] == self.n
+        ncell = conc.shape[1]
+        T = None if temp is None else np.ascontiguousarray(np.broadcast_to(temp, (ncell,)), dtype=np.float64)
+        rho = None if air_density is None else np.ascontiguousarray(np.broadcast_to(air_density, (ncell,)), dtype=np.float64)
+        jv = None
+        if self.nphoto:
+            jv = np.zeros((self.nphoto, ncell)) if jvals is None else np.ascontiguousarray(jvals, dtype=np.float64)
+            assert jv.shape == (self.nphoto, ncell)
+        at = np.ascontiguousarray(np.broadcast_to(1.0e-3 if atol is None else atol, (self.n,)), dtype=np.float64)
+        rt = np.ascontiguousarray(np.broadcast_to(1.0e-6 if rtol is None else rtol, (self.n,)), dtype=np.float64)
+        acc = np.zeros(ncell, dtype=np.int64)
+        att = np.zeros(ncell, dtype=np.int64)
+        stalled = lib().oracle_integrate(self._h, METHODS[method], ncell, _d(conc), _d(T), _d(rho), _d(jv), float(dt),
+                                         int(steps), float(seed), _d(at), _d(rt), _l(acc), _l(att), int(bool(hoist)),
+                                         int(max_attempts), int(threads))
+        return acc, att, stalled
