@@ -1,0 +1,114 @@
+/*
+ * mkpp_amd.h — C ABI of libmkpp_amd.so (every bundled mechanism in one library).
+ *
+ * Three groups of entry points:
+ *
+ * 1. Registry — the C face of the reference's MechanismRegistry
+ *    (src/mkpp_host/include/mkpp_host/mechanism_manager.hpp:28-116:
+ *    get_registered_mechanisms / has_mechanism / get_info).
+ *
+ * 2. Handle API — the reference's per-mechanism C ABI
+ *    (src/mkpp/templates/host_api/mkpp.h.j2:36-72) with the mechanism chosen
+ *    at handle creation: mkpp_amd_X(handle, ...) has exactly the meaning of
+ *    mkpp_X(handle, ...) in mkpp.h.  Extensions without a reference
+ *    counterpart are marked below.
+ *
+ * 3. Device API — stream-ordered entry points on device pointers (no staging,
+ *    no allocation), for hosts that keep the chemistry state resident in HBM.
+ *    No reference counterpart: the reference runs its C ABI on the host only
+ *    (mkpp_c_api.cpp.j2:19) and defers an async API
+ *    (PERFORMANCE_DESIGN_REVIEW.md:26, 68-70).
+ *
+ * Method ids: 0 ROS-2, 1 ROS-3, 2 ROS-4, 3 RODAS-3, 4 RODAS-4
+ * (the reference's --solver choices, src/mkpp/cli.py:394-399).
+ */
+#ifndef MKPP_AMD_H
+#define MKPP_AMD_H
+
+#include <stddef.h>
+
+#include "mkpp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MKPP_METHOD_ROS2 0
+#define MKPP_METHOD_ROS3 1
+#define MKPP_METHOD_ROS4 2
+#define MKPP_METHOD_RODAS3 3
+#define MKPP_METHOD_RODAS4 4
+
+/* Per-cell status codes (src/mkpp_host/include/mkpp_host/dispatcher.hpp:15-21). */
+#define MKPP_CELL_SUCCESS 0
+#define MKPP_CELL_ERR_SINGULAR_MATRIX 1
+#define MKPP_CELL_ERR_NON_FINITE_STATE 2
+#define MKPP_CELL_ERR_CONVERGENCE_FAILURE 3
+#define MKPP_CELL_ERR_NUMERICAL_FAILURE 4
+
+/* ---- 1. registry (mechanism_manager.hpp:28-116) ---- */
+int mkpp_amd_mechanism_count(void);
+const char* mkpp_amd_mechanism_name(int index);
+int mkpp_amd_has_mechanism(const char* name);
+int mkpp_amd_mechanism_info(const char* name, int* num_species, int* num_reactions, int* num_photolysis,
+                            int* num_rate_parameters, int* supports_runtime_environment);
+int mkpp_amd_species_name(const char* mechanism, int index, const char** name_out);
+
+/* ---- 2. handle API (mkpp.h.j2:36-72; mkpp_c_api.cpp.j2:72-371) ---- */
+int mkpp_amd_create_handle(const char* mechanism, mkpp_handle_t* handle_out, size_t num_cells);
+int mkpp_amd_destroy_handle(mkpp_handle_t handle);
+int mkpp_amd_initialize(mkpp_handle_t handle);
+int mkpp_amd_get_species_count(mkpp_handle_t handle, size_t* count_out);
+int mkpp_amd_get_reaction_count(mkpp_handle_t handle, size_t* count_out);
+int mkpp_amd_get_photolysis_count(mkpp_handle_t handle, size_t* count_out);
+int mkpp_amd_get_species_index(mkpp_handle_t handle, const char* species_name, int* index_out);
+int mkpp_amd_get_photolysis_index(mkpp_handle_t handle, const char* photo_name, int* index_out);
+int mkpp_amd_set_state_ptrs(mkpp_handle_t handle, double* species_conc, const double* temp, const double* pressure,
+                            const double* air_density);
+int mkpp_amd_set_rate_parameter_ptrs(mkpp_handle_t handle, const double* values, size_t count);
+int mkpp_amd_set_photolysis_ptrs(mkpp_handle_t handle, const double* photolysis_rates);
+int mkpp_amd_set_cloud_water_ptr(mkpp_handle_t handle, const double* values);
+int mkpp_amd_set_aerosol_ptrs(mkpp_handle_t handle, const double* relative_humidity,
+                              const double* aerosol_surface_area);
+int mkpp_amd_integrate(mkpp_handle_t handle, double dt);
+int mkpp_amd_evaluate_rates(mkpp_handle_t handle, double* rates_out);
+int mkpp_amd_evaluate_jacobian(mkpp_handle_t handle, double* jac_out);
+int mkpp_amd_get_last_error(mkpp_handle_t handle, char* buffer, size_t buffer_len);
+
+/* Extension (no reference counterpart; the reference bakes these into the
+ * generated header: src/mkpp/template_context.py:283-340).  atol/rtol: NULL
+ * keeps the current values, else num_species entries in physical order. */
+int mkpp_amd_set_solver_options(mkpp_handle_t handle, int method, const double* atol, const double* rtol,
+                                double initial_step_seed);
+/* Extension: attempted-step limit per integrate() call (reference: unbounded). */
+int mkpp_amd_set_max_attempts(mkpp_handle_t handle, long max_attempts);
+/* Extension: `steps` consecutive integrate(dt) calls per cell in one launch
+ * (HostExecutionParams::steps, dispatcher.hpp:80-98). */
+int mkpp_amd_integrate_steps(mkpp_handle_t handle, double dt, int steps);
+/* Extension: per-cell status and accepted / attempted step counts of the last
+ * integration (host arrays of num_cells; any may be NULL). */
+int mkpp_amd_get_cell_status(mkpp_handle_t handle, int* status, int* accepted, int* attempted);
+
+/* ---- 3. device API (no reference counterpart) ----
+ * All pointers are device pointers; `stream` is a hipStream_t (NULL = default).
+ * conc[cell + stride*species] (stride >= num_cells), rate_parameters
+ * [cell + parameter_stride*slot] or NULL, temp/air_density [num_cells] or NULL
+ * (300 K, 2.4476e19), atol/rtol [num_species], status/accepted/attempted
+ * [num_cells] or NULL (accepted/attempted are incremented).  Returns after
+ * enqueueing; allocates nothing. */
+int mkpp_amd_device_integrate(const char* mechanism, double* conc, long num_cells, long stride, const double* temp,
+                              const double* air_density, const double* rate_parameters, long parameter_stride,
+                              double dt, int steps, int method, double initial_step_seed, const double* atol,
+                              const double* rtol, int* status, int* accepted, int* attempted, long max_attempts,
+                              void* stream);
+/* what: 0 = rates (out[cell + out_stride*species]),
+ *       1 = dense Jacobian (out[cell + out_stride*(i + num_species*j)]). */
+int mkpp_amd_device_evaluate(const char* mechanism, int what, const double* conc, long num_cells, long stride,
+                             const double* temp, const double* air_density, const double* rate_parameters,
+                             long parameter_stride, double* out, long out_stride, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MKPP_AMD_H */

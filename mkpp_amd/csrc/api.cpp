@@ -1,0 +1,528 @@
+// C-ABI of libmkpp_amd.so: mechanism registry, handle API and device API.
+//
+// The handle API restates the reference's generated C-ABI
+// (src/mkpp/templates/host_api/mkpp_c_api.cpp.j2:40-371): borrowed LayoutLeft
+// host pointers, the same validation and the same return codes.  The
+// per-mechanism drop-in libraries (libmkpp_<mech>.so, capi_shim.cpp) export
+// the reference's unprefixed mkpp_* names on top of these mkpp_amd_* ones.
+// Where the reference runs a host Kokkos parallel_for, this runs one HIP
+// launch on the handle's stream with staging copies in and out; the
+// device-pointer entry points skip the staging.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mkpp_amd.h"
+#include "runtime.hpp"
+
+namespace mkpp {
+
+#include "gen/registry_list.inc"  // declares and lists mkpp_ops_<mech>()
+
+const MechOps* find_mechanism(const char* name) {
+  if (!name) return nullptr;
+  for (int i = 0; i < mechanism_count(); ++i)
+    if (std::strcmp(mechanism_at(i)->name, name) == 0) return mechanism_at(i);
+  return nullptr;
+}
+
+int mechanism_count() { return (int)(sizeof(kRegistry) / sizeof(kRegistry[0])); }
+const MechOps* mechanism_at(int i) { return (i >= 0 && i < mechanism_count()) ? kRegistry[i]() : nullptr; }
+
+}  // namespace mkpp
+
+using mkpp::DeviceArgs;
+using mkpp::EvalArgs;
+using mkpp::MechOps;
+
+struct mkpp_handle_s {
+  const MechOps* ops = nullptr;
+  size_t num_cells = 0;
+  bool initialized = false;
+  double* conc = nullptr;
+  const double* temp = nullptr;
+  const double* pressure = nullptr;
+  const double* air_density = nullptr;
+  const double* params = nullptr;
+  const double* cloud_water = nullptr;
+  const double* rh = nullptr;
+  const double* aerosol_sa = nullptr;
+  // solver options (defaults = the reference's shipped headers: ROS-3,
+  // atol 1e-3, rtol 1e-6, seed 1e-6; template_context.py:283-317)
+  int method = mkpp::ROS3;
+  double seed = 1.0e-6;
+  std::vector<double> atol, rtol;
+  long max_attempts = 1000000;
+  // device staging
+  hipStream_t stream = nullptr;
+  size_t cap_cells = 0;
+  double *d_conc = nullptr, *d_temp = nullptr, *d_rho = nullptr, *d_par = nullptr, *d_tol = nullptr, *d_out = nullptr;
+  size_t cap_out = 0;
+  int *d_status = nullptr, *d_acc = nullptr, *d_att = nullptr;
+  std::vector<int> status, accepted, attempted;
+  std::string last_error;
+
+  void set_error(const std::string& s) { last_error = s; }
+  ~mkpp_handle_s() {
+    hipFree(d_conc); hipFree(d_temp); hipFree(d_rho); hipFree(d_par); hipFree(d_tol); hipFree(d_out);
+    hipFree(d_status); hipFree(d_acc); hipFree(d_att);
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int hip_fail(mkpp_handle_s* h, hipError_t e, const char* what) {
+  if (h) h->set_error(std::string("FATAL ERROR: ") + what + ": " + hipGetErrorString(e));
+  return MKPP_ERROR_INVALID_STATE;
+}
+
+#define HIPCHK(h, call)                                   \
+  do {                                                    \
+    hipError_t e_ = (call);                               \
+    if (e_ != hipSuccess) return hip_fail((h), e_, #call); \
+  } while (0)
+
+// mkpp_c_api.cpp.j2:40-66
+int validate_state(mkpp_handle_s* h) {
+  if (!h->conc) {
+    h->set_error("FATAL ERROR: species concentration pointer is not bound");
+    return MKPP_ERROR_NULL_POINTER;
+  }
+  if (h->num_cells == 0) {
+    h->set_error("FATAL ERROR: species concentration view extents do not match the handle and mechanism");
+    return MKPP_ERROR_INVALID_STATE;
+  }
+  if (!h->temp || !h->air_density || !h->pressure) {
+    h->set_error("FATAL ERROR: per-cell meteorology is not bound");
+    return MKPP_ERROR_NULL_POINTER;
+  }
+  for (size_t c = 0; c < h->num_cells; ++c) {
+    if (!std::isfinite(h->temp[c]) || h->temp[c] <= 0.0 || !std::isfinite(h->air_density[c]) ||
+        h->air_density[c] <= 0.0 || !std::isfinite(h->pressure[c]) || h->pressure[c] <= 0.0) {
+      h->set_error("FATAL ERROR: cell temperature, density and pressure must be positive and finite");
+      return MKPP_ERROR_INVALID_ARGUMENT;
+    }
+  }
+  return MKPP_SUCCESS;
+}
+
+template <class T>
+int grow(mkpp_handle_s* h, T** p, size_t n) {
+  hipFree(*p);
+  *p = nullptr;
+  HIPCHK(h, hipMalloc((void**)p, n * sizeof(T) + 64));
+  return MKPP_SUCCESS;
+}
+
+// Allocate device staging for the handle's cell count (once).
+int ensure_device(mkpp_handle_s* h) {
+  if (!h->stream) HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  if (h->cap_cells >= h->num_cells) return MKPP_SUCCESS;
+  const size_t n = h->num_cells, N = (size_t)h->ops->num_species;
+  const size_t P = (size_t)(h->ops->num_runtime_parameters > 0 ? h->ops->num_runtime_parameters : 1);
+  int rc;
+  if ((rc = grow(h, &h->d_conc, n * N))) return rc;
+  if ((rc = grow(h, &h->d_temp, n))) return rc;
+  if ((rc = grow(h, &h->d_rho, n))) return rc;
+  if ((rc = grow(h, &h->d_par, n * P))) return rc;
+  if ((rc = grow(h, &h->d_tol, 2 * N))) return rc;
+  if ((rc = grow(h, &h->d_status, n))) return rc;
+  if ((rc = grow(h, &h->d_acc, n))) return rc;
+  if ((rc = grow(h, &h->d_att, n))) return rc;
+  h->cap_cells = n;
+  return MKPP_SUCCESS;
+}
+
+int upload_state(mkpp_handle_s* h) {
+  const size_t n = h->num_cells, N = (size_t)h->ops->num_species;
+  const size_t P = (size_t)h->ops->num_runtime_parameters;
+  HIPCHK(h, hipMemcpyAsync(h->d_conc, h->conc, n * N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->d_temp, h->temp, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->d_rho, h->air_density, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  if (P > 0 && h->params)
+    HIPCHK(h, hipMemcpyAsync(h->d_par, h->params, n * P * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  return MKPP_SUCCESS;
+}
+
+mkpp_handle_s* H(mkpp_handle_t h) { return reinterpret_cast<mkpp_handle_s*>(h); }
+
+}  // namespace
+
+extern "C" {
+
+/* ---------------- registry ---------------- */
+int mkpp_amd_mechanism_count(void) { return mkpp::mechanism_count(); }
+
+const char* mkpp_amd_mechanism_name(int index) {
+  const MechOps* o = mkpp::mechanism_at(index);
+  return o ? o->name : nullptr;
+}
+
+int mkpp_amd_has_mechanism(const char* name) { return mkpp::find_mechanism(name) != nullptr; }
+
+int mkpp_amd_mechanism_info(const char* name, int* num_species, int* num_reactions, int* num_photolysis,
+                            int* num_rate_parameters, int* supports_runtime_environment) {
+  const MechOps* o = mkpp::find_mechanism(name);
+  if (!name) return MKPP_ERROR_NULL_POINTER;
+  if (!o) return MKPP_ERROR_NOT_FOUND;
+  if (num_species) *num_species = o->num_species;
+  if (num_reactions) *num_reactions = o->num_reactions;
+  if (num_photolysis) *num_photolysis = o->num_photolysis;
+  if (num_rate_parameters) *num_rate_parameters = o->num_runtime_parameters;
+  if (supports_runtime_environment) *supports_runtime_environment = o->supports_runtime_environment ? 1 : 0;
+  return MKPP_SUCCESS;
+}
+
+int mkpp_amd_species_name(const char* mechanism, int index, const char** name_out) {
+  const MechOps* o = mkpp::find_mechanism(mechanism);
+  if (!mechanism || !name_out) return MKPP_ERROR_NULL_POINTER;
+  if (!o) return MKPP_ERROR_NOT_FOUND;
+  if (index < 0 || index >= o->num_species) return MKPP_ERROR_INVALID_ARGUMENT;
+  *name_out = o->species_names[index];
+  return MKPP_SUCCESS;
+}
+
+/* ---------------- handle lifecycle ---------------- */
+int mkpp_amd_create_handle(const char* mechanism, mkpp_handle_t* handle_out, size_t num_cells) {
+  if (!handle_out || !mechanism) return MKPP_ERROR_NULL_POINTER;
+  if (num_cells == 0) return MKPP_ERROR_INVALID_ARGUMENT;
+  const MechOps* o = mkpp::find_mechanism(mechanism);
+  if (!o) return MKPP_ERROR_NOT_FOUND;
+  try {
+    auto* h = new mkpp_handle_s();
+    h->ops = o;
+    h->num_cells = num_cells;
+    h->atol.assign(o->num_species, 1.0e-3);
+    h->rtol.assign(o->num_species, 1.0e-6);
+    *handle_out = reinterpret_cast<mkpp_handle_t>(h);
+    return MKPP_SUCCESS;
+  } catch (...) {
+    return MKPP_ERROR_INVALID_STATE;
+  }
+}
+
+int mkpp_amd_destroy_handle(mkpp_handle_t handle) {
+  if (!handle) return MKPP_ERROR_NULL_POINTER;
+  delete H(handle);
+  return MKPP_SUCCESS;
+}
+
+int mkpp_amd_initialize(mkpp_handle_t handle) {
+  if (!handle) return MKPP_ERROR_NULL_POINTER;
+  H(handle)->initialized = true;
+  return MKPP_SUCCESS;
+}
+
+int mkpp_amd_get_species_count(mkpp_handle_t handle, size_t* count_out) {
+  if (!handle || !count_out) return MKPP_ERROR_NULL_POINTER;
+  *count_out = (size_t)H(handle)->ops->num_species;
+  return MKPP_SUCCESS;
+}
+
+int mkpp_amd_get_reaction_count(mkpp_handle_t handle, size_t* count_out) {
+  if (!handle || !count_out) return MKPP_ERROR_NULL_POINTER;
+  *count_out = (size_t)H(handle)->ops->num_reactions;
+  return MKPP_SUCCESS;
+}
+
+int mkpp_amd_get_photolysis_count(mkpp_handle_t handle, size_t* count_out) {
+  if (!handle || !count_out) return MKPP_ERROR_NULL_POINTER;
+  *count_out = (size_t)H(handle)->ops->num_photolysis;
+  return MKPP_SUCCESS;
+}
+
+int mkpp_amd_get_species_index(mkpp_handle_t handle, const char* species_name, int* index_out) {
+  if (!handle || !species_name || !index_out) return MKPP_ERROR_NULL_POINTER;
+  const MechOps* o = H(handle)->ops;
+  for (int i = 0; i < o->num_species; ++i)
+    if (std::strcmp(o->species_names[i], species_name) == 0) {
+      *index_out = i;
+      return MKPP_SUCCESS;
+    }
+  *index_out = -1;
+  return MKPP_ERROR_NOT_FOUND;
+}
+
+// The reference's photolysis name map falls back to "J_<1-based index>"
+// because its photolysis records carry no name (mkpp_c_api.cpp.j2:31-37).
+int mkpp_amd_get_photolysis_index(mkpp_handle_t handle, const char* photo_name, int* index_out) {
+  if (!handle || !photo_name || !index_out) return MKPP_ERROR_NULL_POINTER;
+  const MechOps* o = H(handle)->ops;
+  char buf[32];
+  for (int i = 0; i < o->num_photolysis; ++i) {
+    std::snprintf(buf, sizeof buf, "J_%d", i + 1);
+    if (std::strcmp(buf, photo_name) == 0) {
+      *index_out = i;
+      return MKPP_SUCCESS;
+    }
+  }
+  *index_out = -1;
+  return MKPP_ERROR_NOT_FOUND;
+}
+
+/* ---------------- host memory binding ---------------- */
+int mkpp_amd_set_state_ptrs(mkpp_handle_t handle, double* species_conc, const double* temp, const double* pressure,
+                            const double* air_density) {
+  if (!handle) return MKPP_ERROR_NULL_POINTER;
+  mkpp_handle_s* h = H(handle);
+  if (!species_conc || !temp || !pressure || !air_density) {
+    h->set_error("FATAL ERROR: state binding requires concentration and meteorology pointers");
+    return MKPP_ERROR_NULL_POINTER;
+  }
+  h->conc = species_conc;
+  h->temp = temp;
+  h->pressure = pressure;
+  h->air_density = air_density;
+  return validate_state(h);
+}
+
+int mkpp_amd_set_rate_parameter_ptrs(mkpp_handle_t handle, const double* values, size_t count) {
+  if (!handle || !values) return MKPP_ERROR_NULL_POINTER;
+  mkpp_handle_s* h = H(handle);
+  if (count != (size_t)h->ops->num_runtime_parameters) {
+    h->set_error("FATAL ERROR: runtime parameter count does not match the generated mechanism");
+    return MKPP_ERROR_INVALID_ARGUMENT;
+  }
+  h->params = values;
+  return MKPP_SUCCESS;
+}
+
+int mkpp_amd_set_photolysis_ptrs(mkpp_handle_t handle, const double* photolysis_rates) {
+  if (!handle || !photolysis_rates) return MKPP_ERROR_NULL_POINTER;
+  mkpp_handle_s* h = H(handle);
+  if (h->ops->num_runtime_parameters > h->ops->num_photolysis) {
+    h->set_error("FATAL ERROR: use mkpp_set_rate_parameter_ptrs for the complete runtime parameter array");
+    return MKPP_ERROR_INVALID_ARGUMENT;
+  }
+  return mkpp_amd_set_rate_parameter_ptrs(handle, photolysis_rates, (size_t)h->ops->num_photolysis);
+}
+
+int mkpp_amd_set_cloud_water_ptr(mkpp_handle_t handle, const double* values) {
+  if (!handle || !values) return MKPP_ERROR_NULL_POINTER;
+  H(handle)->cloud_water = values;
+  return MKPP_SUCCESS;
+}
+
+int mkpp_amd_set_aerosol_ptrs(mkpp_handle_t handle, const double* relative_humidity, const double* aerosol_surface_area) {
+  if (!handle) return MKPP_ERROR_NULL_POINTER;
+  if (relative_humidity) H(handle)->rh = relative_humidity;
+  if (aerosol_surface_area) H(handle)->aerosol_sa = aerosol_surface_area;
+  return MKPP_SUCCESS;
+}
+
+/* ---------------- solver options (no reference counterpart) ---------------- */
+int mkpp_amd_set_solver_options(mkpp_handle_t handle, int method, const double* atol, const double* rtol,
+                                double initial_step_seed) {
+  if (!handle) return MKPP_ERROR_NULL_POINTER;
+  mkpp_handle_s* h = H(handle);
+  if (method < mkpp::ROS2 || method > mkpp::RODAS4) {
+    h->set_error("FATAL ERROR: unknown Rosenbrock method");
+    return MKPP_ERROR_INVALID_ARGUMENT;
+  }
+  if (!std::isfinite(initial_step_seed) || initial_step_seed <= 0.0) {
+    h->set_error("FATAL ERROR: initial_step_seed must be a finite positive fraction of the chemistry interval");
+    return MKPP_ERROR_INVALID_ARGUMENT;
+  }
+  h->method = method;
+  h->seed = initial_step_seed;
+  if (atol) h->atol.assign(atol, atol + h->ops->num_species);
+  if (rtol) h->rtol.assign(rtol, rtol + h->ops->num_species);
+  return MKPP_SUCCESS;
+}
+
+int mkpp_amd_set_max_attempts(mkpp_handle_t handle, long max_attempts) {
+  if (!handle) return MKPP_ERROR_NULL_POINTER;
+  H(handle)->max_attempts = max_attempts;
+  return MKPP_SUCCESS;
+}
+
+/* ---------------- computation ---------------- */
+int mkpp_amd_integrate_steps(mkpp_handle_t handle, double dt, int steps) {
+  if (!handle) return MKPP_ERROR_NULL_POINTER;
+  mkpp_handle_s* h = H(handle);
+  if (!h->initialized) {
+    h->set_error("FATAL ERROR: MKPP handle must be initialized before integration");
+    return MKPP_ERROR_INVALID_STATE;
+  }
+  int rc = validate_state(h);
+  if (rc != MKPP_SUCCESS) return rc;
+  if (!std::isfinite(dt) || dt <= 0.0 || steps < 1) return MKPP_ERROR_INVALID_ARGUMENT;
+  if ((rc = ensure_device(h)) != MKPP_SUCCESS) return rc;
+  if ((rc = upload_state(h)) != MKPP_SUCCESS) return rc;
+  const size_t n = h->num_cells, N = (size_t)h->ops->num_species;
+  std::vector<double> tol(2 * N);
+  for (size_t i = 0; i < N; ++i) {
+    tol[i] = h->atol[i];
+    tol[N + i] = h->rtol[i];
+  }
+  HIPCHK(h, hipMemcpyAsync(h->d_tol, tol.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemsetAsync(h->d_acc, 0, n * sizeof(int), h->stream));
+  HIPCHK(h, hipMemsetAsync(h->d_att, 0, n * sizeof(int), h->stream));
+  DeviceArgs a{};
+  a.conc = h->d_conc;
+  a.ncells = (long)n;
+  a.stride = (long)n;
+  a.temp = h->d_temp;
+  a.air_density = h->d_rho;
+  a.jvals = (h->params && h->ops->num_runtime_parameters > 0) ? h->d_par : nullptr;
+  a.jstride = (long)n;
+  a.dt = dt;
+  a.steps = steps;
+  a.method = h->method;
+  a.seed = h->seed;
+  a.atol = h->d_tol;
+  a.rtol = h->d_tol + N;
+  a.status = h->d_status;
+  a.accepted = h->d_acc;
+  a.attempted = h->d_att;
+  a.max_attempts = h->max_attempts;
+  if (h->ops->integrate(a, h->stream) != 0) {
+    h->set_error("FATAL ERROR: integrate kernel launch failed");
+    return MKPP_ERROR_INVALID_STATE;
+  }
+  h->status.resize(n);
+  h->accepted.resize(n);
+  h->attempted.resize(n);
+  HIPCHK(h, hipMemcpyAsync(h->conc, h->d_conc, n * N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->status.data(), h->d_status, n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->accepted.data(), h->d_acc, n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->attempted.data(), h->d_att, n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (size_t c = 0; c < n; ++c) {
+    if (h->status[c] == mkpp::ERR_CONVERGENCE_FAILURE) {
+      h->set_error("FATAL ERROR: step-size control exceeded the attempted-step limit in cell " + std::to_string(c));
+      return MKPP_ERROR_CONVERGENCE;
+    }
+  }
+  return MKPP_SUCCESS;
+}
+
+int mkpp_amd_integrate(mkpp_handle_t handle, double dt) { return mkpp_amd_integrate_steps(handle, dt, 1); }
+
+int mkpp_amd_get_cell_status(mkpp_handle_t handle, int* status, int* accepted, int* attempted) {
+  if (!handle) return MKPP_ERROR_NULL_POINTER;
+  mkpp_handle_s* h = H(handle);
+  if (h->status.size() != h->num_cells) {
+    h->set_error("FATAL ERROR: no integration has run on this handle");
+    return MKPP_ERROR_INVALID_STATE;
+  }
+  if (status) std::memcpy(status, h->status.data(), h->num_cells * sizeof(int));
+  if (accepted) std::memcpy(accepted, h->accepted.data(), h->num_cells * sizeof(int));
+  if (attempted) std::memcpy(attempted, h->attempted.data(), h->num_cells * sizeof(int));
+  return MKPP_SUCCESS;
+}
+
+static int evaluate(mkpp_handle_t handle, double* out, bool jac) {
+  if (!handle) return MKPP_ERROR_NULL_POINTER;
+  mkpp_handle_s* h = H(handle);
+  if (!out) {
+    h->set_error(jac ? "FATAL ERROR: Jacobian output pointer is null" : "FATAL ERROR: rates output pointer is null");
+    return MKPP_ERROR_NULL_POINTER;
+  }
+  if (!h->initialized) {
+    h->set_error(jac ? "FATAL ERROR: MKPP handle must be initialized before Jacobian evaluation"
+                     : "FATAL ERROR: MKPP handle must be initialized before rates evaluation");
+    return MKPP_ERROR_INVALID_STATE;
+  }
+  int rc = validate_state(h);
+  if (rc != MKPP_SUCCESS) return rc;
+  if ((rc = ensure_device(h)) != MKPP_SUCCESS) return rc;
+  if ((rc = upload_state(h)) != MKPP_SUCCESS) return rc;
+  const size_t n = h->num_cells, N = (size_t)h->ops->num_species;
+  const size_t nout = jac ? n * N * N : n * N;
+  if (h->cap_out < nout) {
+    if ((rc = grow(h, &h->d_out, nout))) return rc;
+    h->cap_out = nout;
+  }
+  EvalArgs e{};
+  e.conc = h->d_conc;
+  e.ncells = (long)n;
+  e.stride = (long)n;
+  e.temp = h->d_temp;
+  e.air_density = h->d_rho;
+  e.jvals = (h->params && h->ops->num_runtime_parameters > 0) ? h->d_par : nullptr;
+  e.jstride = (long)n;
+  e.out = h->d_out;
+  e.out_stride = (long)n;
+  if ((jac ? h->ops->eval_jacobian(e, h->stream) : h->ops->eval_rates(e, h->stream)) != 0) {
+    h->set_error("FATAL ERROR: evaluation kernel launch failed");
+    return MKPP_ERROR_INVALID_STATE;
+  }
+  HIPCHK(h, hipMemcpyAsync(out, h->d_out, nout * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return MKPP_SUCCESS;
+}
+
+int mkpp_amd_evaluate_rates(mkpp_handle_t handle, double* rates_out) { return evaluate(handle, rates_out, false); }
+int mkpp_amd_evaluate_jacobian(mkpp_handle_t handle, double* jac_out) { return evaluate(handle, jac_out, true); }
+
+int mkpp_amd_get_last_error(mkpp_handle_t handle, char* buffer, size_t buffer_len) {
+  if (!handle || !buffer || buffer_len == 0) return MKPP_ERROR_NULL_POINTER;
+  std::strncpy(buffer, H(handle)->last_error.c_str(), buffer_len - 1);
+  buffer[buffer_len - 1] = '\0';
+  return MKPP_SUCCESS;
+}
+
+/* ---------------- device-pointer API ---------------- */
+int mkpp_amd_device_integrate(const char* mechanism, double* conc, long num_cells, long stride, const double* temp,
+                              const double* air_density, const double* rate_parameters, long parameter_stride,
+                              double dt, int steps, int method, double initial_step_seed, const double* atol,
+                              const double* rtol, int* status, int* accepted, int* attempted, long max_attempts,
+                              void* stream) {
+  const MechOps* o = mkpp::find_mechanism(mechanism);
+  if (!mechanism || !conc || !atol || !rtol) return MKPP_ERROR_NULL_POINTER;
+  if (!o) return MKPP_ERROR_NOT_FOUND;
+  if (num_cells < 0 || stride < num_cells || !std::isfinite(dt) || dt <= 0.0 || steps < 1 ||
+      method < mkpp::ROS2 || method > mkpp::RODAS4 || !(initial_step_seed > 0.0))
+    return MKPP_ERROR_INVALID_ARGUMENT;
+  if (rate_parameters && parameter_stride < num_cells) return MKPP_ERROR_INVALID_ARGUMENT;
+  DeviceArgs a{};
+  a.conc = conc;
+  a.ncells = num_cells;
+  a.stride = stride;
+  a.temp = temp;
+  a.air_density = air_density;
+  a.jvals = rate_parameters;
+  a.jstride = parameter_stride;
+  a.dt = dt;
+  a.steps = steps;
+  a.method = method;
+  a.seed = initial_step_seed;
+  a.atol = atol;
+  a.rtol = rtol;
+  a.status = status;
+  a.accepted = accepted;
+  a.attempted = attempted;
+  a.max_attempts = max_attempts;
+  return o->integrate(a, (hipStream_t)stream) == 0 ? MKPP_SUCCESS : MKPP_ERROR_INVALID_STATE;
+}
+
+int mkpp_amd_device_evaluate(const char* mechanism, int what, const double* conc, long num_cells, long stride,
+                             const double* temp, const double* air_density, const double* rate_parameters,
+                             long parameter_stride, double* out, long out_stride, void* stream) {
+  const MechOps* o = mkpp::find_mechanism(mechanism);
+  if (!mechanism || !conc || !out) return MKPP_ERROR_NULL_POINTER;
+  if (!o) return MKPP_ERROR_NOT_FOUND;
+  if (num_cells < 0 || stride < num_cells || out_stride < num_cells || (what != 0 && what != 1))
+    return MKPP_ERROR_INVALID_ARGUMENT;
+  EvalArgs e{};
+  e.conc = conc;
+  e.ncells = num_cells;
+  e.stride = stride;
+  e.temp = temp;
+  e.air_density = air_density;
+  e.jvals = rate_parameters;
+  e.jstride = parameter_stride;
+  e.out = out;
+  e.out_stride = out_stride;
+  const int rc = what == 0 ? o->eval_rates(e, (hipStream_t)stream) : o->eval_jacobian(e, (hipStream_t)stream);
+  return rc == 0 ? MKPP_SUCCESS : MKPP_ERROR_INVALID_STATE;
+}
+
+}  // extern "C"

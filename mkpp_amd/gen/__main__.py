@@ -24,6 +24,7 @@ def main(argv=None):
     root = planmod.ROOT
     os.makedirs(os.path.join(root, "mkpp_amd", "plans"), exist_ok=True)
     os.makedirs(os.path.join(root, "oracle", "tables"), exist_ok=True)
+    compiled = []
     for name in args.mechanisms:
         mech = load_mechanism(name)
         p = planmod.build(mech)
@@ -33,9 +34,20 @@ def main(argv=None):
             from . import emit_hip
 
             emit_hip.write(mech, p, os.path.join(root, "mkpp_amd", "csrc", "gen"))
+        if not args.no_hip and name in BUNDLED:
+            compiled.append(name)
         print(f"{name}: N={len(p['species'])} groups={len(p['groups'])} J={len(p['j_entries'])} "
               f"LU={len(p['lu_locations'])} step_flops(rodas3)={p['step_flops']['rodas3']}", file=sys.stderr)
+    if compiled:
+        registry([m for m in BUNDLED if m in compiled])
+
+
+def registry(names):
+    from . import emit_hip
+
+    return emit_hip.write_registry(names, os.path.join(planmod.ROOT, "mkpp_amd", "csrc", "gen"))
 
 
 if __name__ == "__main__":
     main()
+

@@ -1,0 +1,251 @@
+"""ctypes binding of libmkpp_amd.so and the per-mechanism drop-in libraries.
+
+Nothing here falls back to Python or CPU code: if the native library is
+missing the import of a function raises.  Arrays use the C ABI's LayoutLeft
+order, i.e. numpy/torch arrays shaped (num_species, num_cells), C-contiguous,
+so that element [species, cell] sits at cell + num_cells * species.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+
+MKPP_SUCCESS = 0
+ERRORS = {
+    -101: "MKPP_ERROR_NULL_POINTER",
+    -102: "MKPP_ERROR_INVALID_HANDLE",
+    -103: "MKPP_ERROR_INVALID_ARGUMENT",
+    -104: "MKPP_ERROR_INVALID_STATE",
+    -105: "MKPP_ERROR_CONVERGENCE",
+    -106: "MKPP_ERROR_NOT_FOUND",
+}
+METHODS = {"ros2": 0, "ros3": 1, "ros4": 2, "rodas3": 3, "rodas4": 4}
+
+
+class MkppError(RuntimeError):
+    def __init__(self, code: int, msg: str = ""):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_vp, _dp, _ip, _sz = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_size_t
+
+_SIGS = {
+    "mkpp_amd_mechanism_count": ([], C.c_int),
+    "mkpp_amd_mechanism_name": ([C.c_int], C.c_char_p),
+    "mkpp_amd_has_mechanism": ([C.c_char_p], C.c_int),
+    "mkpp_amd_mechanism_info": ([C.c_char_p, _ip, _ip, _ip, _ip, _ip], C.c_int),
+    "mkpp_amd_species_name": ([C.c_char_p, C.c_int, C.POINTER(C.c_char_p)], C.c_int),
+    "mkpp_amd_create_handle": ([C.c_char_p, C.POINTER(_vp), _sz], C.c_int),
+    "mkpp_amd_destroy_handle": ([_vp], C.c_int),
+    "mkpp_amd_initialize": ([_vp], C.c_int),
+    "mkpp_amd_get_species_count": ([_vp, C.POINTER(_sz)], C.c_int),
+    "mkpp_amd_get_reaction_count": ([_vp, C.POINTER(_sz)], C.c_int),
+    "mkpp_amd_get_photolysis_count": ([_vp, C.POINTER(_sz)], C.c_int),
+    "mkpp_amd_get_species_index": ([_vp, C.c_char_p, _ip], C.c_int),
+    "mkpp_amd_get_photolysis_index": ([_vp, C.c_char_p, _ip], C.c_int),
+    "mkpp_amd_set_state_ptrs": ([_vp, _dp, _dp, _dp, _dp], C.c_int),
+    "mkpp_amd_set_rate_parameter_ptrs": ([_vp, _dp, _sz], C.c_int),
+    "mkpp_amd_set_photolysis_ptrs": ([_vp, _dp], C.c_int),
+    "mkpp_amd_integrate": ([_vp, C.c_double], C.c_int),
+    "mkpp_amd_integrate_steps": ([_vp, C.c_double, C.c_int], C.c_int),
+    "mkpp_amd_evaluate_rates": ([_vp, _dp], C.c_int),
+    "mkpp_amd_evaluate_jacobian": ([_vp, _dp], C.c_int),
+    "mkpp_amd_get_last_error": ([_vp, C.c_char_p, _sz], C.c_int),
+    "mkpp_amd_set_solver_options": ([_vp, C.c_int, _dp, _dp, C.c_double], C.c_int),
+    "mkpp_amd_set_max_attempts": ([_vp, C.c_long], C.c_int),
+    "mkpp_amd_get_cell_status": ([_vp, _ip, _ip, _ip], C.c_int),
+    "mkpp_amd_device_integrate": ([C.c_char_p, _vp, C.c_long, C.c_long, _vp, _vp, _vp, C.c_long, C.c_double, C.c_int,
+                                   C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, C.c_long, _vp], C.c_int),
+    "mkpp_amd_device_evaluate": ([C.c_char_p, C.c_int, _vp, C.c_long, C.c_long, _vp, _vp, _vp, C.c_long, _vp, C.c_long,
+                                  _vp], C.c_int),
+}
+
+_lib = None
+
+
+def library_path() -> str:
+    return os.path.join(LIBDIR, "libmkpp_amd.so")
+
+
+def lib():
+    """Load libmkpp_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is missing: build it with `python -m mkpp_amd.build`")
+        L = C.CDLL(path, mode=C.RTLD_GLOBAL)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def dropin_path(mechanism: str) -> str:
+    return os.path.join(LIBDIR, f"libmkpp_{mechanism}.so")
+
+
+def load_dropin(mechanism: str):
+    """The reference-named C ABI (mkpp_*) of one mechanism."""
+    lib()  # resolve libmkpp_amd.so first
+    path = dropin_path(mechanism)
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m mkpp_amd.build`")
+    return C.CDLL(path)
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def mechanisms() -> list[str]:
+    L = lib()
+    return [L.mkpp_amd_mechanism_name(i).decode() for i in range(L.mkpp_amd_mechanism_count())]
+
+
+def mechanism_info(name: str) -> dict:
+    vals = [C.c_int() for _ in range(5)]
+    rc = lib().mkpp_amd_mechanism_info(name.encode(), *[C.byref(v) for v in vals])
+    if rc != MKPP_SUCCESS:
+        raise MkppError(rc, name)
+    keys = ("num_species", "num_reactions", "num_photolysis", "num_rate_parameters", "supports_runtime_environment")
+    info = {k: v.value for k, v in zip(keys, vals)}
+    names = []
+    for i in range(info["num_species"]):
+        p = C.c_char_p()
+        lib().mkpp_amd_species_name(name.encode(), i, C.byref(p))
+        names.append(p.value.decode())
+    info["species"] = names
+    return info
+
+
+class Handle:
+    """Handle API over host numpy buffers (borrowed, like the reference's C ABI)."""
+
+    def __init__(self, mechanism: str, num_cells: int):
+        self.mechanism = mechanism
+        self.num_cells = num_cells
+        self._h = _vp()
+        self._check(lib().mkpp_amd_create_handle(mechanism.encode(), C.byref(self._h), num_cells), "create")
+        self._check(lib().mkpp_amd_initialize(self._h), "initialize")
+        self.info = mechanism_info(mechanism)
+        self._keep = []
+
+    def _check(self, rc, what):
+        if rc != MKPP_SUCCESS:
+            msg = ""
+            if getattr(self, "_h", None) and self._h.value:
+                buf = C.create_string_buffer(512)
+                lib().mkpp_amd_get_last_error(self._h, buf, 512)
+                msg = buf.value.decode()
+            raise MkppError(rc, f"{what}: {msg}")
+
+    def close(self):
+        if self._h.value:
+            lib().mkpp_amd_destroy_handle(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bind(self, conc: np.ndarray, temp, pressure, air_density, rate_parameters=None):
+        n = self.info["num_species"]
+        assert conc.dtype == np.float64 and conc.flags.c_contiguous and conc.shape == (n, self.num_cells)
+        arrs = [np.ascontiguousarray(np.broadcast_to(x, (self.num_cells,)), dtype=np.float64)
+                for x in (temp, pressure, air_density)]
+        self._keep = [conc, *arrs]
+        self._check(lib().mkpp_amd_set_state_ptrs(self._h, _d(conc), *[_d(a) for a in arrs]), "set_state_ptrs")
+        if rate_parameters is not None:
+            rp = np.ascontiguousarray(rate_parameters, dtype=np.float64)
+            assert rp.shape == (self.info["num_rate_parameters"], self.num_cells)
+            self._keep.append(rp)
+            self._check(lib().mkpp_amd_set_rate_parameter_ptrs(self._h, _d(rp), rp.shape[0]), "set_rate_parameter_ptrs")
+
+    def set_solver(self, method="ros3", atol=None, rtol=None, seed=1.0e-6):
+        n = self.info["num_species"]
+        at = None if atol is None else np.ascontiguousarray(np.broadcast_to(atol, (n,)), dtype=np.float64)
+        rt = None if rtol is None else np.ascontiguousarray(np.broadcast_to(rtol, (n,)), dtype=np.float64)
+        self._check(lib().mkpp_amd_set_solver_options(self._h, METHODS[method], _d(at), _d(rt), float(seed)), "set_solver_options")
+
+    def set_max_attempts(self, n: int):
+        self._check(lib().mkpp_amd_set_max_attempts(self._h, int(n)), "set_max_attempts")
+
+    def integrate(self, dt: float, steps: int = 1):
+        self._check(lib().mkpp_amd_integrate_steps(self._h, float(dt), int(steps)), "integrate")
+
+    def cell_status(self):
+        st = np.zeros(self.num_cells, np.int32)
+        acc = np.zeros(self.num_cells, np.int32)
+        att = np.zeros(self.num_cells, np.int32)
+        ip = C.POINTER(C.c_int)
+        self._check(lib().mkpp_amd_get_cell_status(self._h, st.ctypes.data_as(ip), acc.ctypes.data_as(ip),
+                                                     att.ctypes.data_as(ip)), "get_cell_status")
+        return st, acc, att
+
+    def evaluate_rates(self):
+        out = np.zeros((self.info["num_species"], self.num_cells))
+        self._check(lib().mkpp_amd_evaluate_rates(self._h, _d(out)), "evaluate_rates")
+        return out
+
+    def evaluate_jacobian(self):
+        n = self.info["num_species"]
+        out = np.zeros((n, n, self.num_cells))  # [j][i][cell] == cell + ncells*(i + n*j)
+        self._check(lib().mkpp_amd_evaluate_jacobian(self._h, _d(out)), "evaluate_jacobian")
+        return out.transpose(1, 0, 2)  # -> [i][j][cell]
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def device_integrate(mechanism: str, conc, dt: float, steps: int = 1, method: str = "rodas3", temp=None,
+                     air_density=None, rate_parameters=None, atol=None, rtol=None, seed: float = 1.0e-6,
+                     status=None, accepted=None, attempted=None, max_attempts: int = 1000000, stream=None):
+    """Integrate device-resident cells in place (torch CUDA tensors, float64).
+
+    conc: (num_species, num_cells) contiguous; rate_parameters: (P, num_cells);
+    atol/rtol: (num_species,) device tensors.  Enqueued on `stream` (default:
+    torch's current stream); returns immediately.
+    """
+    import torch
+
+    n, ncell = conc.shape
+    assert conc.is_cuda and conc.dtype == torch.float64 and conc.is_contiguous()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    rc = lib().mkpp_amd_device_integrate(
+        mechanism.encode(), _ptr(conc), ncell, ncell, _ptr(temp), _ptr(air_density), _ptr(rate_parameters),
+        ncell if rate_parameters is not None else 0, float(dt), int(steps), METHODS[method], float(seed),
+        _ptr(atol), _ptr(rtol), _ptr(status), _ptr(accepted), _ptr(attempted), int(max_attempts),
+        C.c_void_p(s.cuda_stream))
+    if rc != MKPP_SUCCESS:
+        raise MkppError(rc, "device_integrate")
+
+
+def device_evaluate(mechanism: str, what: str, conc, temp=None, air_density=None, rate_parameters=None, stream=None):
+    import torch
+
+    n, ncell = conc.shape
+    if what == "rates":
+        out = torch.empty((n, ncell), dtype=torch.float64, device=conc.device)
+    else:
+        out = torch.empty((n, n, ncell), dtype=torch.float64, device=conc.device)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    rc = lib().mkpp_amd_device_evaluate(mechanism.encode(), 0 if what == "rates" else 1, _ptr(conc), ncell, ncell,
+                                        _ptr(temp), _ptr(air_density), _ptr(rate_parameters),
+                                        ncell if rate_parameters is not None else 0, _ptr(out), ncell,
+                                        C.c_void_p(s.cuda_stream))
+    if rc != MKPP_SUCCESS:
+        raise MkppError(rc, "device_evaluate")
+    return out if what == "rates" else out.permute(1, 0, 2)
