@@ -57,7 +57,7 @@ struct mkpp_handle_s {
   int method = mkpp::ROS3;
   double seed = 1.0e-6;
   std::vector<double> atol, rtol;
-  long max_attempts = 1000000;
+  long max_attempts = 100000;  // KPP Rosenbrock default Max_no_steps
   // device staging
   hipStream_t stream = nullptr;
   size_t cap_cells = 0;

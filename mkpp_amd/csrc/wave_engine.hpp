@@ -1,0 +1,574 @@
+// Wave-per-cell Rosenbrock engine for mechanisms too large for one lane
+// (saprc99: 78 variable species, 1547 LU entries, 9654 products per factor).
+//
+// One 64-lane wavefront integrates one cell; a grid-stride loop hands the
+// wavefront its next cell.  LDS holds the W/LU matrix in row-displacement
+// storage (element (i, j) at R[i] + j), the stage state y for F gathers and a
+// scratch area for fluxes / Jacobian partials.  Stage vectors live in
+// registers, lane = row (bank b covers rows 64b .. 64b+63).
+//
+// Arithmetic follows the oracle's order per entry: F_i and J_ij accumulate in
+// flux-group order, every LU entry subtracts its products in ascending pivot
+// order then divides by the pivot (right-looking schedule, same sequence per
+// entry as the reference's Crout-ordered plan), the error norm sums rows in
+// logical order.  Two differences, both within rounding: LU, solve and
+// accumulation updates use fused multiply-add, and the backward substitution
+// subtracts a row's products in descending column order.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mkpp_device.hpp"
+#include "runtime.hpp"
+
+namespace mkpp {
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// Rate constant of one flux group (src/mkpp/lowering.py:199-313); P holds the
+// law's numeric parameters in the order of mkpp_amd.gen.plan.LAW_PARAMS.
+__device__ __forceinline__ double wave_rate(int law, const double* P, double T, double M, double jv) {
+  switch (law) {
+    case 0:  // PHOTOLYSIS
+      return jv;
+    case 1: {  // ARRHENIUS: A (T/300)^B exp(C/T); unit factors skipped exactly
+      double k = P[0];
+      k = (P[1] != 0.0) ? k * pow(T / 300.0, P[1]) : k * 1.0;
+      if (P[2] != 0.0) k = k * exp(P[2] / T);
+      return k;
+    }
+    case 2: {  // TROE: A0 B0 C0 A1 B1 C1 Fc
+      const double k0 = P[0] * exp(P[2] / T) * pow(T / 300.0, P[1]);
+      const double ki = P[3] * exp(P[5] / T) * pow(T / 300.0, P[4]);
+      const double K0 = k0 * M;
+      const double r = K0 / ki;
+      const double lg = log10(r);
+      const double ex = 1.0 / (1.0 + lg * lg);
+      return K0 / (1.0 + r) * pow(P[6], ex);
+    }
+    case 3: {  // EP2: A0 C0 A2 C2 A3 C3
+      const double K0 = P[0] * exp(P[1] / T);
+      const double K2 = P[2] * exp(P[3] / T);
+      const double K3 = P[4] * exp(P[5] / T) * M;
+      return K0 + K3 / (1.0 + K3 / K2);
+    }
+    case 4: {  // EP3: A1 C1 A2 C2
+      const double K1 = P[0] * exp(P[1] / T);
+      const double K2 = P[2] * exp(P[3] / T);
+      return K1 + K2 * M;
+    }
+    case 5:  // HETEROGENEOUS: 0.25 gamma (v_gas = S_a = 1)
+      return 0.25 * P[0];
+    default:  // CONSTANT
+      return P[0];
+  }
+}
+
+template <class W>
+struct WaveTables;  // specialised per mechanism by MKPP_WAVE_MECHANISM
+
+// bank-select without dynamic register-array indexing
+template <int NB>
+__device__ __forceinline__ double bank(const double (&v)[NB], int row) {
+  if constexpr (NB == 1) {
+    return v[0];
+  } else {
+    return row < 64 ? v[0] : v[1];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The fused integrate kernel.
+// ---------------------------------------------------------------------------
+template <class W, class Tab>
+__global__ void __launch_bounds__(64) wave_integrate_kernel(DeviceArgs a) {
+  using T_ = WaveTables<W>;
+  constexpr int NV = W::NV, NB = W::NB, NG = W::NG, NGP = (NG + 63) / 64, S = Tab::S;
+  extern __shared__ double lds[];
+  double* lu = lds;                 // [SPAN]
+  double* ylds = lds + W::SPAN;     // [NY]: variable (logical), fixed, ONE
+  double* scr = ylds + W::NY;       // [NSCR]: fluxes or partials
+  const int lane = threadIdx.x;
+
+  int rb[NB];
+  bool valid[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int row = b * 64 + lane;
+    valid[b] = row < NV;
+    rb[b] = valid[b] ? (int)T_::R[row] : 0;
+  }
+
+  for (long cell = blockIdx.x; cell < a.ncells; cell += gridDim.x) {
+    // ---------------- load ----------------
+    double x[NB], atol[NB], rtol[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int row = b * 64 + lane;
+      const int ph = valid[b] ? (int)T_::VARPHYS[row] : 0;
+      x[b] = valid[b] ? a.conc[cell + a.stride * ph] : 0.0;
+      atol[b] = valid[b] ? a.atol[ph] : 1.0;
+      rtol[b] = valid[b] ? a.rtol[ph] : 0.0;
+    }
+    if (lane < W::NF) ylds[NV + lane] = a.conc[cell + a.stride * (long)T_::FIXPHYS[lane]];
+    if (lane == 0) ylds[W::ONE] = 1.0;
+    const double Tk = a.temp ? a.temp[cell] : 300.0;
+    const double rho = a.air_density ? a.air_density[cell] : 2.4476e19;
+    const double Mtb = (W::THIRD_BODY >= 0) ? a.conc[cell + a.stride * (long)(W::THIRD_BODY >= 0 ? W::THIRD_BODY : 0)] : rho;
+
+    // non-finite pre-check (dispatcher.hpp:289-307) when a status array is given
+    if (a.status != nullptr) {
+      bool ok = true;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) ok = ok && isfinite(x[b]);
+      if (lane < W::NF) ok = ok && isfinite(ylds[NV + lane]);
+      const unsigned long long bad = __ballot(!ok);
+      if (bad != 0ull) {
+        if (lane == 0) a.status[cell] = ERR_NON_FINITE_STATE;
+        __syncthreads();
+        continue;
+      }
+    }
+
+    // ---------------- rate constants (once per cell and call) ----------------
+    double kreg[NGP];
+#pragma unroll
+    for (int p = 0; p < NGP; ++p) {
+      const int g = p * 64 + lane;
+      kreg[p] = 0.0;
+      if (g < NG) {
+        const int law = T_::LAW[g];
+        const int ph = T_::PHOTO[g];
+        const double jv = (law == 0 && a.jvals != nullptr) ? a.jvals[cell + a.jstride * ph] : 0.0;
+        kreg[p] = wave_rate(law, &T_::PARAMS[g * 7], Tk, Mtb, jv);
+      }
+    }
+    __syncthreads();
+
+    // F at the state in ylds: fluxes by group lanes, then rows by row lanes
+    auto eval_rates = [&](double (&f)[NB]) {
+#pragma unroll
+      for (int p = 0; p < NGP; ++p) {
+        const int g = p * 64 + lane;
+        if (g < NG) {
+          const uint8_t* fa = &T_::GFAC[g * 3];
+          scr[g] = kreg[p] * (((1.0 * ylds[fa[0]]) * ylds[fa[1]]) * ylds[fa[2]]);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        double acc = 0.0;
+        if (valid[b]) {
+          const int row = b * 64 + lane;
+          const int t1 = T_::FOFS[row + 1];
+          for (int t = T_::FOFS[row]; t < t1; ++t) acc = acc + T_::COEF[T_::FC[t]] * scr[T_::FG[t]];
+        }
+        f[b] = acc;
+      }
+      __syncthreads();
+    };
+
+    // W = I/(g h) - J(y) into the LU storage
+    auto assemble_w = [&](double inv_g_dt) {
+#pragma unroll
+      for (int p = 0; p < NGP; ++p) {
+        const int g = p * 64 + lane;
+        if (g < NG) {
+          const int np = T_::GNPOW[g];
+          const uint8_t* s = &T_::GPOWS[g * 3];
+          const uint8_t* q = &T_::GPOWP[g * 3];
+          const int off = T_::POFF[g];
+          for (int e = 0; e < np; ++e) {
+            // d flux / d y_e: k * p * y_e^(p-1) * prod_{other entries} y^p
+            double t = kreg[p] * (double)q[e];
+            double ip = 1.0;
+            for (int r = 1; r < q[e]; ++r) ip = (r == 1) ? ylds[s[e]] : ip * ylds[s[e]];
+            t = t * ip;
+            for (int o = 0; o < np; ++o) {
+              if (o == e) continue;
+              double po = ylds[s[o]];
+              for (int r = 1; r < q[o]; ++r) po = po * ylds[s[o]];
+              t = t * po;
+            }
+            scr[off + e] = t;
+          }
+        }
+      }
+      __syncthreads();
+      for (int e0 = 0; e0 < W::NENTRIES; e0 += 64) {
+        const int e = e0 + lane;
+        if (e < W::NENTRIES) {
+          double acc = 0.0;
+          const int t1 = T_::WOFS[e + 1];
+          for (int t = T_::WOFS[e]; t < t1; ++t) acc = acc + T_::COEF[T_::WCI[t]] * scr[T_::WPI[t]];
+          lu[T_::WADDR[e]] = (T_::WDIAG[e] ? inv_g_dt : 0.0) - acc;
+        }
+      }
+      __syncthreads();
+    };
+
+    // right-looking LU in place
+    auto factor = [&]() {
+      for (int k = 0; k < NV; ++k) {
+        const int cnt = T_::LCNT[k];
+        if (cnt == 0) continue;
+        const int rk = T_::R[k];
+        const double d = lu[rk + k];
+        if (lane < cnt) {
+          const int i = T_::LROW[T_::LOFS[k] + lane];
+          const int ri = T_::R[i];
+          const double l = lu[ri + k] / d;
+          lu[ri + k] = l;
+          const int u1 = T_::UOFS[k + 1];
+          for (int u = T_::UOFS[k]; u < u1; ++u) {
+            const int j = T_::UCOL[u];
+            lu[ri + j] = fma(-l, lu[rk + j], lu[ri + j]);
+          }
+        }
+        __syncthreads();
+      }
+    };
+
+    // forward/backward substitution, column oriented, lane = row
+    auto solve = [&](const double (&rhs)[NB], double (&xs)[NB]) {
+      double w[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) w[b] = rhs[b];
+      for (int k = 0; k < NV; ++k) {
+        const double xk = readlane_d(bank<NB>(w, k), k & 63);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const unsigned long long m = T_::FMASK[k * NB + b];
+          if ((m >> lane) & 1ull) w[b] = fma(-lu[rb[b] + k], xk, w[b]);
+        }
+      }
+      for (int k = NV - 1; k >= 0; --k) {
+        const double xk = readlane_d(bank<NB>(w, k), k & 63) / lu[T_::R[k] + k];
+        if ((k & 63) == lane) {
+          if (k < 64) xs[0] = xk;
+          else xs[NB - 1] = xk;
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const unsigned long long m = T_::BMASK[k * NB + b];
+          if ((m >> lane) & 1ull) w[b] = fma(-lu[rb[b] + k], xk, w[b]);
+        }
+      }
+    };
+
+    auto store_y = [&](const double (&y)[NB]) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        if (valid[b]) ylds[b * 64 + lane] = y[b];
+      __syncthreads();
+    };
+
+    // ---------------- integrate `steps` times over dt ----------------
+    int status = SUCCESS;
+    int acc_steps = 0, att_steps = 0;
+    for (int call = 0; call < a.steps && status == SUCCESS; ++call) {
+      const double dt_total = a.dt;
+      const double h_min = dt_total * 1.0e-15;
+      double t = 0.0, dt = dt_total * a.seed;
+      bool reject_last = false, reject_more = false;
+      long attempts = 0;
+      while (t < dt_total) {
+        if (a.max_attempts > 0 && attempts >= a.max_attempts) {
+          status = ERR_CONVERGENCE_FAILURE;
+          break;
+        }
+        ++attempts;
+        ++att_steps;
+        dt = fmin(dt, dt_total - t);
+        const double inv_g_dt = 1.0 / (Tab::G * dt);
+        double Sx[NB], K[S][NB], F[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) Sx[b] = x[b];
+        store_y(Sx);
+        assemble_w(inv_g_dt);
+        factor();
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          double rhs[NB];
+          const int base = s * (s - 1) / 2;
+          if (s == 0) {
+            eval_rates(F);
+          } else if (Tab::NEWF[s]) {
+            double Y[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+              double v = Sx[b];
+#pragma unroll
+              for (int j = 0; j < s; ++j) {
+                const double c = Tab::A[base + j];
+                if (c == 0.0) continue;
+                v = (c == 1.0) ? v + K[j][b] : v + c * K[j][b];
+              }
+              Y[b] = v;
+            }
+            store_y(Y);
+            eval_rates(F);
+          }
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            double v = F[b];
+            if (s > 0) {
+#pragma unroll
+              for (int j = 0; j < s; ++j) {
+                const double c = Tab::C[base + j];
+                if (c == 0.0) continue;
+                v = v + (c / dt) * K[j][b];
+              }
+            }
+            rhs[b] = v;
+          }
+          solve(rhs, K[s]);
+        }
+        // error norm: rows summed in logical order (fixed species add zeros)
+        double term[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          double ynew = Sx[b], yerr = 0.0;
+          bool first = true;
+#pragma unroll
+          for (int j = 0; j < S; ++j) {
+            const double m = Tab::M[j];
+            if (m == 0.0) continue;
+            ynew = ynew + ((m == 1.0) ? K[j][b] : m * K[j][b]);
+          }
+#pragma unroll
+          for (int j = 0; j < S; ++j) {
+            const double e = Tab::E[j];
+            if (e == 0.0) continue;
+            const double tt = (e == 1.0) ? K[j][b] : e * K[j][b];
+            yerr = first ? tt : yerr + tt;
+            first = false;
+          }
+          const double sci = atol[b] + rtol[b] * fmax(fabs(x[b]), fabs(ynew));
+          term[b] = valid[b] ? (yerr / sci) * (yerr / sci) : 0.0;
+        }
+        double err_sq = 0.0;
+        for (int i = 0; i < NV; ++i) err_sq += readlane_d(bank<NB>(term, i), i & 63);
+        double err = sqrt(err_sq / W::N);
+        err = fmax(err, 1.0e-10);
+        const double factor_ = step_factor<Tab>(err);
+        if (err <= 1.0) {
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            double msum = 0.0;
+            bool first = true;
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+              const double m = Tab::M[j];
+              if (m == 0.0) continue;
+              const double tt = (m == 1.0) ? K[j][b] : m * K[j][b];
+              msum = first ? tt : msum + tt;
+              first = false;
+            }
+            x[b] += msum;
+          }
+          t += dt;
+          ++acc_steps;
+          double next_dt = fmax(h_min, fmin(dt * factor_, dt_total));
+          if (reject_last) next_dt = fmin(next_dt, dt);
+          reject_last = false;
+          reject_more = false;
+          dt = next_dt;
+        } else {
+          dt *= reject_more ? 0.1 : factor_;
+          reject_more = reject_last;
+          reject_last = true;
+        }
+      }
+    }
+    // ---------------- store ----------------
+    bool fin = true;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      if (valid[b]) a.conc[cell + a.stride * (long)T_::VARPHYS[b * 64 + lane]] = x[b];
+      fin = fin && (!valid[b] || isfinite(x[b]));
+    }
+    const bool all_fin = __ballot(!fin) == 0ull;
+    if (lane == 0) {
+      if (status == SUCCESS && !all_fin) status = ERR_NUMERICAL_FAILURE;
+      if (a.status) a.status[cell] = status;
+      if (a.accepted) a.accepted[cell] += acc_steps;
+      if (a.attempted) a.attempted[cell] += att_steps;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Evaluation kernels (one thread per cell; not on the hot path).
+// ---------------------------------------------------------------------------
+template <class W>
+__global__ void __launch_bounds__(64) wave_eval_kernel(EvalArgs a, int what) {
+  using T_ = WaveTables<W>;
+  const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= a.ncells) return;
+  const double Tk = a.temp ? a.temp[cell] : 300.0;
+  const double rho = a.air_density ? a.air_density[cell] : 2.4476e19;
+  const double Mtb = (W::THIRD_BODY >= 0) ? a.conc[cell + a.stride * (long)(W::THIRD_BODY >= 0 ? W::THIRD_BODY : 0)] : rho;
+  auto y = [&](int phys) { return T_::UNIT[phys] ? 1.0 : a.conc[cell + a.stride * phys]; };
+  if (what == 0) {
+    for (int i = 0; i < W::N; ++i) a.out[cell + a.out_stride * i] = 0.0;
+  } else {
+    for (int e = 0; e < W::N * W::N; ++e) a.out[cell + a.out_stride * (long)e] = 0.0;
+  }
+  for (int g = 0; g < W::NG; ++g) {
+    const int law = T_::LAW[g];
+    const double jv = (law == 0 && a.jvals != nullptr) ? a.jvals[cell + a.jstride * T_::PHOTO[g]] : 0.0;
+    const double* P = &T_::PARAMS[g * 7];
+    const double k = wave_rate(law, P, Tk, Mtb, jv);
+    const int np = T_::GNPOW_FULL[g];
+    const uint8_t* s = &T_::GPOWS_FULL[g * 3];
+    const uint8_t* q = &T_::GPOWP[g * 3];
+    const int n0 = T_::GNUOFS[g], n1 = T_::GNUOFS[g + 1];
+    if (what == 0) {
+      double mono = 1.0;
+      for (int e = 0; e < np; ++e) {
+        double v = y(s[e]);
+        for (int r = 1; r < q[e]; ++r) v = v * y(s[e]);
+        mono = mono * v;
+      }
+      const double flux = k * mono;
+      for (int t = n0; t < n1; ++t) {
+        const long o = cell + a.out_stride * (long)T_::GNUS[t];
+        a.out[o] = a.out[o] + T_::COEF[T_::GNUC[t]] * flux;
+      }
+    } else {
+      // dk/d[M] for the third-body laws
+      double dk = 0.0;
+      if (law == 2) {
+        const double k0 = P[0] * exp(P[2] / Tk) * pow(Tk / 300.0, P[1]);
+        const double ki = P[3] * exp(P[5] / Tk) * pow(Tk / 300.0, P[4]);
+        const double K0 = k0 * Mtb, r = K0 / ki, lg = log10(r), ex = 1.0 / (1.0 + lg * lg), Fb = pow(P[6], ex);
+        const double dr = k0 / ki, dex = -ex * ex * 2.0 * lg / (r * log(10.0)) * dr, dF = Fb * log(P[6]) * dex;
+        dk = (k0 / (1.0 + r) - K0 * dr / ((1.0 + r) * (1.0 + r))) * Fb + K0 / (1.0 + r) * dF;
+      } else if (law == 3) {
+        const double K2 = P[2] * exp(P[3] / Tk), k3 = P[4] * exp(P[5] / Tk), K3 = k3 * Mtb;
+        dk = k3 / ((1.0 + K3 / K2) * (1.0 + K3 / K2));
+      } else if (law == 4) {
+        dk = P[2] * exp(P[3] / Tk);
+      }
+      const int d0 = T_::GDEPOFS[g], d1 = T_::GDEPOFS[g + 1];
+      for (int d = d0; d < d1; ++d) {
+        const int j = T_::GDEP[d];
+        double part = 0.0;
+        for (int e = 0; e < np; ++e) {
+          if (s[e] != j) continue;
+          double term = k * (double)q[e];
+          double ip = 1.0;
+          for (int r = 1; r < q[e]; ++r) ip = (r == 1) ? y(j) : ip * y(j);
+          term = term * ip;
+          for (int o = 0; o < np; ++o) {
+            if (o == e) continue;
+            double v = y(s[o]);
+            for (int r = 1; r < q[o]; ++r) v = v * y(s[o]);
+            term = term * v;
+          }
+          part = part + term;
+        }
+        if (law >= 2 && law <= 4 && j == W::THIRD_BODY) {
+          double term = dk;
+          for (int o = 0; o < np; ++o) {
+            double v = y(s[o]);
+            for (int r = 1; r < q[o]; ++r) v = v * y(s[o]);
+            term = term * v;
+          }
+          part = part + term;
+        }
+        for (int t = n0; t < n1; ++t) {
+          const long o = cell + a.out_stride * (long)(T_::GNUS[t] + W::N * j);
+          a.out[o] = a.out[o] + T_::COEF[T_::GNUC[t]] * part;
+        }
+      }
+    }
+  }
+}
+
+template <class W>
+int launch_wave_integrate(const DeviceArgs& a, hipStream_t s) {
+  if (a.ncells <= 0) return 0;
+  const size_t lds = sizeof(double) * W::LDS_DOUBLES;
+  const unsigned grid = (unsigned)(a.ncells < (1L << 30) ? a.ncells : (1L << 30));
+  switch (a.method) {
+    case ROS2: wave_integrate_kernel<W, TabRos2><<<grid, 64, lds, s>>>(a); break;
+    case ROS3: wave_integrate_kernel<W, TabRos3><<<grid, 64, lds, s>>>(a); break;
+    case ROS4: wave_integrate_kernel<W, TabRos4><<<grid, 64, lds, s>>>(a); break;
+    case RODAS3: wave_integrate_kernel<W, TabRodas3><<<grid, 64, lds, s>>>(a); break;
+    case RODAS4: wave_integrate_kernel<W, TabRodas4><<<grid, 64, lds, s>>>(a); break;
+    default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+template <class W>
+int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
+  if (a.ncells <= 0) return 0;
+  wave_eval_kernel<W><<<dim3((unsigned)((a.ncells + 63) / 64)), dim3(64), 0, s>>>(a, what);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace mkpp
+
+#define MKPP_WAVE_MECHANISM(NAME, F0, F1, F2, F3, F4)                                                         \
+  namespace mkpp {                                                                                          \
+  template <>                                                                                               \
+  struct WaveTables<gen::NAME##_wave::Wave> {                                                               \
+    static constexpr const int16_t* R = gen::NAME##_wave::R;                                               \
+    static constexpr const uint8_t* LCNT = gen::NAME##_wave::LCNT;                                          \
+    static constexpr const uint16_t* LOFS = gen::NAME##_wave::LOFS;                                         \
+    static constexpr const uint8_t* LROW = gen::NAME##_wave::LROW;                                          \
+    static constexpr const uint16_t* UOFS = gen::NAME##_wave::UOFS;                                         \
+    static constexpr const uint8_t* UCOL = gen::NAME##_wave::UCOL;                                          \
+    static constexpr const uint64_t* FMASK = gen::NAME##_wave::FMASK;                                       \
+    static constexpr const uint64_t* BMASK = gen::NAME##_wave::BMASK;                                       \
+    static constexpr const uint8_t* LAW = gen::NAME##_wave::LAW;                                            \
+    static constexpr const int16_t* PHOTO = gen::NAME##_wave::PHOTO;                                        \
+    static constexpr const double* PARAMS = gen::NAME##_wave::PARAMS;                                       \
+    static constexpr const uint8_t* GFAC = gen::NAME##_wave::GFAC;                                          \
+    static constexpr const uint8_t* GNPOW = gen::NAME##_wave::GNPOW;                                        \
+    static constexpr const uint8_t* GPOWS = gen::NAME##_wave::GPOWS;                                        \
+    static constexpr const uint8_t* GPOWP = gen::NAME##_wave::GPOWP;                                        \
+    static constexpr const uint16_t* POFF = gen::NAME##_wave::POFF;                                         \
+    static constexpr const double* COEF = gen::NAME##_wave::COEF;                                           \
+    static constexpr const uint16_t* FOFS = gen::NAME##_wave::FOFS;                                         \
+    static constexpr const uint8_t* FG = gen::NAME##_wave::FG;                                              \
+    static constexpr const uint8_t* FC = gen::NAME##_wave::FC;                                              \
+    static constexpr const uint16_t* WADDR = gen::NAME##_wave::WADDR;                                       \
+    static constexpr const uint8_t* WDIAG = gen::NAME##_wave::WDIAG;                                        \
+    static constexpr const uint16_t* WOFS = gen::NAME##_wave::WOFS;                                         \
+    static constexpr const uint16_t* WPI = gen::NAME##_wave::WPI;                                           \
+    static constexpr const uint8_t* WCI = gen::NAME##_wave::WCI;                                            \
+    static constexpr const uint8_t* VARPHYS = gen::NAME##_wave::VARPHYS;                                    \
+    static constexpr const uint8_t* FIXPHYS = gen::NAME##_wave::FIXPHYS;                                    \
+    static constexpr const uint8_t* UNIT = gen::NAME##_wave::UNIT;                                          \
+    static constexpr const uint8_t* GNPOW_FULL = gen::NAME##_wave::GNPOW;                                   \
+    static constexpr const uint8_t* GPOWS_FULL = gen::NAME##_wave::GPOWS_PHYS;                              \
+    static constexpr const uint16_t* GNUOFS = gen::NAME##_wave::GNUOFS;                                     \
+    static constexpr const uint8_t* GNUS = gen::NAME##_wave::GNUS;                                          \
+    static constexpr const uint8_t* GNUC = gen::NAME##_wave::GNUC;                                          \
+    static constexpr const uint16_t* GDEPOFS = gen::NAME##_wave::GDEPOFS;                                   \
+    static constexpr const uint8_t* GDEP = gen::NAME##_wave::GDEP;                                          \
+  };                                                                                                        \
+  static int wave_eval_rates_##NAME(const EvalArgs& a, hipStream_t s) {                                     \
+    return launch_wave_eval<gen::NAME##_wave::Wave>(a, s, 0);                                               \
+  }                                                                                                         \
+  static int wave_eval_jac_##NAME(const EvalArgs& a, hipStream_t s) {                                       \
+    return launch_wave_eval<gen::NAME##_wave::Wave>(a, s, 1);                                               \
+  }                                                                                                         \
+  const MechOps* mkpp_ops_##NAME() {                                                                        \
+    using W = gen::NAME##_wave::Wave;                                                                       \
+    static const MechOps ops = {#NAME, W::N, W::NUM_REACTIONS, W::NPH, W::NPH, 0,                          \
+                                W::SUPPORTS_RUNTIME_ENVIRONMENT, W::SPECIES,                                \
+                                &launch_wave_integrate<W>, &wave_eval_rates_##NAME, &wave_eval_jac_##NAME, \
+                                {F0, F1, F2, F3, F4}};                                                      \
+    return &ops;                                                                                            \
+  }                                                                                                         \
+  }

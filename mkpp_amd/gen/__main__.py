@@ -31,9 +31,14 @@ def main(argv=None):
         planmod.write_plan_json(p, os.path.join(root, "mkpp_amd", "plans", f"{name}.json"))
         planmod.write_oracle_table(p, os.path.join(root, "oracle", "tables", f"{name}.tab"))
         if not args.no_hip and name in BUNDLED:
-            from . import emit_hip
+            from . import emit_hip, emit_wave
 
-            emit_hip.write(mech, p, os.path.join(root, "mkpp_amd", "csrc", "gen"))
+            outdir = os.path.join(root, "mkpp_amd", "csrc", "gen")
+            nv = len(p["species"]) - sum(p["fixed"])
+            if nv <= emit_hip.LANE_MAX_VARIABLE_SPECIES:
+                emit_hip.write(mech, p, outdir)
+            else:
+                emit_wave.write(mech, p, outdir)
         if not args.no_hip and name in BUNDLED:
             compiled.append(name)
         print(f"{name}: N={len(p['species'])} groups={len(p['groups'])} J={len(p['j_entries'])} "

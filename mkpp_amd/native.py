@@ -212,7 +212,7 @@ def _ptr(t):
 
 def device_integrate(mechanism: str, conc, dt: float, steps: int = 1, method: str = "rodas3", temp=None,
                      air_density=None, rate_parameters=None, atol=None, rtol=None, seed: float = 1.0e-6,
-                     status=None, accepted=None, attempted=None, max_attempts: int = 1000000, stream=None):
+                     status=None, accepted=None, attempted=None, max_attempts: int = 100000, stream=None):
     """Integrate device-resident cells in place (torch CUDA tensors, float64).
 
     conc: (num_species, num_cells) contiguous; rate_parameters: (P, num_cells);
