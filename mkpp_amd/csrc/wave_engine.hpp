@@ -82,15 +82,23 @@ __device__ __forceinline__ double bank(const double (&v)[NB], int row) {
 
 // ---------------------------------------------------------------------------
 // The fused integrate kernel.
+//
+// LDS layout per cell (one wavefront):
+//   lu   [SPAN + 1]  W / LU factors, row displacement; slot SPAN absorbs padding
+//   ylds [NY]        stage state gathered by the flux lanes (+ fixed species, 1.0)
+//   scr  [NSCR]      fluxes (F) or Jacobian partials (W); last used slot = 0.0
+//   facc [NV + 1]    F row accumulators; slot NV absorbs padding
 // ---------------------------------------------------------------------------
 template <class W, class Tab>
-__global__ void __launch_bounds__(64) wave_integrate_kernel(DeviceArgs a) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) wave_integrate_kernel(DeviceArgs a) {
   using T_ = WaveTables<W>;
+  using C_ = typename T_::Code;
   constexpr int NV = W::NV, NB = W::NB, NG = W::NG, NGP = (NG + 63) / 64, S = Tab::S;
   extern __shared__ double lds[];
-  double* lu = lds;                 // [SPAN]
-  double* ylds = lds + W::SPAN;     // [NY]: variable (logical), fixed, ONE
-  double* scr = ylds + W::NY;       // [NSCR]: fluxes or partials
+  double* lu = lds;
+  double* ylds = lu + (W::SPAN + 1);
+  double* scr = ylds + W::NY;
+  double* facc = scr + W::NSCR;
   const int lane = threadIdx.x;
 
   int rb[NB];
@@ -125,10 +133,8 @@ __global__ void __launch_bounds__(64) wave_integrate_kernel(DeviceArgs a) {
 #pragma unroll
       for (int b = 0; b < NB; ++b) ok = ok && isfinite(x[b]);
       if (lane < W::NF) ok = ok && isfinite(ylds[NV + lane]);
-      const unsigned long long bad = __ballot(!ok);
-      if (bad != 0ull) {
+      if (__ballot(!ok) != 0ull) {
         if (lane == 0) a.status[cell] = ERR_NON_FINITE_STATE;
-        __syncthreads();
         continue;
       }
     }
@@ -146,9 +152,9 @@ __global__ void __launch_bounds__(64) wave_integrate_kernel(DeviceArgs a) {
         kreg[p] = wave_rate(law, &T_::PARAMS[g * 7], Tk, Mtb, jv);
       }
     }
-    __syncthreads();
 
-    // F at the state in ylds: fluxes by group lanes, then rows by row lanes
+    // F at the state in ylds: fluxes by group lanes, then fixed-size term
+    // chunks accumulated per row (chunks of one row sit in successive passes)
     auto eval_rates = [&](double (&f)[NB]) {
 #pragma unroll
       for (int p = 0; p < NGP; ++p) {
@@ -158,18 +164,22 @@ __global__ void __launch_bounds__(64) wave_integrate_kernel(DeviceArgs a) {
           scr[g] = kreg[p] * (((1.0 * ylds[fa[0]]) * ylds[fa[1]]) * ylds[fa[2]]);
         }
       }
-      __syncthreads();
+      if (lane == 0) scr[NG] = 0.0;
+      for (int r = lane; r <= NV; r += 64) facc[r] = 0.0;
 #pragma unroll
-      for (int b = 0; b < NB; ++b) {
+      for (int p = 0; p < C_::NFP; ++p) {
         double acc = 0.0;
-        if (valid[b]) {
-          const int row = b * 64 + lane;
-          const int t1 = T_::FOFS[row + 1];
-          for (int t = T_::FOFS[row]; t < t1; ++t) acc = acc + T_::COEF[T_::FC[t]] * scr[T_::FG[t]];
+#pragma unroll
+        for (int q = 0; q < C_::TF; ++q) {
+          const uint32_t tm = T_::CF_TERM[(p * C_::TF + q) * 64 + lane];
+          acc = fma(T_::COEF[tm >> 16], scr[tm & 0xFFFFu], acc);
         }
-        f[b] = acc;
+        const int row = T_::CF_ROW[p * 64 + lane];
+        const int r = row == 255 ? NV : row;
+        facc[r] = facc[r] + acc;
       }
-      __syncthreads();
+#pragma unroll
+      for (int b = 0; b < NB; ++b) f[b] = valid[b] ? facc[b * 64 + lane] : 0.0;
     };
 
     // W = I/(g h) - J(y) into the LU storage
@@ -183,7 +193,7 @@ __global__ void __launch_bounds__(64) wave_integrate_kernel(DeviceArgs a) {
           const uint8_t* q = &T_::GPOWP[g * 3];
           const int off = T_::POFF[g];
           for (int e = 0; e < np; ++e) {
-            // d flux / d y_e: k * p * y_e^(p-1) * prod_{other entries} y^p
+            // d flux / d y_e = k * p * y_e^(p-1) * prod_{other entries} y^p
             double t = kreg[p] * (double)q[e];
             double ip = 1.0;
             for (int r = 1; r < q[e]; ++r) ip = (r == 1) ? ylds[s[e]] : ip * ylds[s[e]];
@@ -198,65 +208,22 @@ __global__ void __launch_bounds__(64) wave_integrate_kernel(DeviceArgs a) {
           }
         }
       }
-      __syncthreads();
-      for (int e0 = 0; e0 < W::NENTRIES; e0 += 64) {
-        const int e = e0 + lane;
-        if (e < W::NENTRIES) {
-          double acc = 0.0;
-          const int t1 = T_::WOFS[e + 1];
-          for (int t = T_::WOFS[e]; t < t1; ++t) acc = acc + T_::COEF[T_::WCI[t]] * scr[T_::WPI[t]];
-          lu[T_::WADDR[e]] = (T_::WDIAG[e] ? inv_g_dt : 0.0) - acc;
-        }
-      }
-      __syncthreads();
-    };
-
-    // right-looking LU in place
-    auto factor = [&]() {
-      for (int k = 0; k < NV; ++k) {
-        const int cnt = T_::LCNT[k];
-        if (cnt == 0) continue;
-        const int rk = T_::R[k];
-        const double d = lu[rk + k];
-        if (lane < cnt) {
-          const int i = T_::LROW[T_::LOFS[k] + lane];
-          const int ri = T_::R[i];
-          const double l = lu[ri + k] / d;
-          lu[ri + k] = l;
-          const int u1 = T_::UOFS[k + 1];
-          for (int u = T_::UOFS[k]; u < u1; ++u) {
-            const int j = T_::UCOL[u];
-            lu[ri + j] = fma(-l, lu[rk + j], lu[ri + j]);
-          }
-        }
-        __syncthreads();
-      }
-    };
-
-    // forward/backward substitution, column oriented, lane = row
-    auto solve = [&](const double (&rhs)[NB], double (&xs)[NB]) {
-      double w[NB];
+      if (lane == 0) scr[W::NPART] = 0.0;
+      for (int i = lane; i <= W::SPAN; i += 64) lu[i] = 0.0;
 #pragma unroll
-      for (int b = 0; b < NB; ++b) w[b] = rhs[b];
-      for (int k = 0; k < NV; ++k) {
-        const double xk = readlane_d(bank<NB>(w, k), k & 63);
+      for (int b = 0; b < NB; ++b)
+        if (valid[b]) lu[rb[b] + b * 64 + lane] = inv_g_dt;
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          const unsigned long long m = T_::FMASK[k * NB + b];
-          if ((m >> lane) & 1ull) w[b] = fma(-lu[rb[b] + k], xk, w[b]);
-        }
-      }
-      for (int k = NV - 1; k >= 0; --k) {
-        const double xk = readlane_d(bank<NB>(w, k), k & 63) / lu[T_::R[k] + k];
-        if ((k & 63) == lane) {
-          if (k < 64) xs[0] = xk;
-          else xs[NB - 1] = xk;
-        }
+      for (int p = 0; p < C_::NWP; ++p) {
+        double acc = 0.0;
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          const unsigned long long m = T_::BMASK[k * NB + b];
-          if ((m >> lane) & 1ull) w[b] = fma(-lu[rb[b] + k], xk, w[b]);
+        for (int q = 0; q < C_::TW; ++q) {
+          const uint32_t tm = T_::CW_TERM[(p * C_::TW + q) * 64 + lane];
+          acc = fma(T_::COEF[tm >> 16], scr[tm & 0xFFFFu], acc);
         }
+        const int ad = T_::CW_ADDR[p * 64 + lane];
+        const int adr = ad == 0xFFFF ? W::SPAN : ad;
+        lu[adr] = lu[adr] - acc;
       }
     };
 
@@ -264,7 +231,6 @@ __global__ void __launch_bounds__(64) wave_integrate_kernel(DeviceArgs a) {
 #pragma unroll
       for (int b = 0; b < NB; ++b)
         if (valid[b]) ylds[b * 64 + lane] = y[b];
-      __syncthreads();
     };
 
     // ---------------- integrate `steps` times over dt ----------------
@@ -285,15 +251,17 @@ __global__ void __launch_bounds__(64) wave_integrate_kernel(DeviceArgs a) {
         ++att_steps;
         dt = fmin(dt, dt_total - t);
         const double inv_g_dt = 1.0 / (Tab::G * dt);
-        double Sx[NB], K[S][NB], F[NB];
+        double Sx[NB], K[S][NB], F[NB], rdiag[NB];
 #pragma unroll
         for (int b = 0; b < NB; ++b) Sx[b] = x[b];
         store_y(Sx);
         assemble_w(inv_g_dt);
-        factor();
+        C_::factor(lu, rb, lane);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) rdiag[b] = valid[b] ? 1.0 / lu[rb[b] + b * 64 + lane] : 0.0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-          double rhs[NB];
+          double wv[NB];
           const int base = s * (s - 1) / 2;
           if (s == 0) {
             eval_rates(F);
@@ -324,9 +292,10 @@ __global__ void __launch_bounds__(64) wave_integrate_kernel(DeviceArgs a) {
                 v = v + (c / dt) * K[j][b];
               }
             }
-            rhs[b] = v;
+            wv[b] = valid[b] ? v : 0.0;
           }
-          solve(rhs, K[s]);
+          C_::forward(lu, rb, wv, lane);
+          C_::backward(lu, rb, wv, rdiag, K[s], lane);
         }
         // error norm: rows summed in logical order (fixed species add zeros)
         double term[NB];
@@ -399,7 +368,6 @@ __global__ void __launch_bounds__(64) wave_integrate_kernel(DeviceArgs a) {
       if (a.accepted) a.accepted[cell] += acc_steps;
       if (a.attempted) a.attempted[cell] += att_steps;
     }
-    __syncthreads();
   }
 }
 
@@ -556,6 +524,11 @@ int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
     static constexpr const uint8_t* GNUC = gen::NAME##_wave::GNUC;                                          \
     static constexpr const uint16_t* GDEPOFS = gen::NAME##_wave::GDEPOFS;                                   \
     static constexpr const uint8_t* GDEP = gen::NAME##_wave::GDEP;                                          \
+    static constexpr const uint32_t* CF_TERM = gen::NAME##_wave::CF_TERM;                                   \
+    static constexpr const uint8_t* CF_ROW = gen::NAME##_wave::CF_ROW;                                      \
+    static constexpr const uint32_t* CW_TERM = gen::NAME##_wave::CW_TERM;                                   \
+    static constexpr const uint16_t* CW_ADDR = gen::NAME##_wave::CW_ADDR;                                   \
+    using Code = gen::NAME##_wave::Code;                                                                    \
   };                                                                                                        \
   static int wave_eval_rates_##NAME(const EvalArgs& a, hipStream_t s) {                                     \
     return launch_wave_eval<gen::NAME##_wave::Wave>(a, s, 0);                                               \

@@ -266,8 +266,8 @@ def emit(t: dict) -> str:
         f"  static constexpr int NPART = {t['npart']}, ONE = {t['one']}, NENTRIES = {t['nentries']};",
         f"  static constexpr int THIRD_BODY = {t['third_body']}, NUM_REACTIONS = {t['num_reactions']};",
         f"  static constexpr bool SUPPORTS_RUNTIME_ENVIRONMENT = {'true' if t['supports_runtime_environment'] else 'false'};",
-        "  static constexpr int NSCR = (NG > NPART ? NG : NPART);",
-        "  static constexpr int LDS_DOUBLES = SPAN + NY + NSCR;",
+        "  static constexpr int NSCR = (NG > NPART ? NG : NPART) + 1;  // + a 0.0 padding slot",
+        "  static constexpr int LDS_DOUBLES = (SPAN + 1) + NY + NSCR + (NV + 1);",
         "  static constexpr const char* SPECIES[N] = {" + ", ".join(f'"{s}"' for s in t["species"]) + "};",
         "};",
         "",
@@ -282,14 +282,15 @@ def emit_unit(t: dict) -> str:
     flops = ", ".join(f"{float(sf[m])!r}" for m in ("ros2", "ros3", "ros4", "rodas3", "rodas4"))
     return (
         f"// Generated by mkpp_amd.gen.emit_wave: registers {name} (one cell per wavefront).\n"
-        f'#include "{name}_wave.hpp"\n'
-        '#include "../wave_engine.hpp"\n\n'
+        '#include "../wave_engine.hpp"\n'
+        f'#include "{name}_wcode.hpp"\n\n'
         f"MKPP_WAVE_MECHANISM({name}, {flops})\n"
     )
 
 
 def write(mech: Mechanism, p: dict, outdir: str) -> list:
     t = build_tables(mech, p)
+    ct = build_code_tables(t)  # may append 0.0 to the coefficient table
     os.makedirs(outdir, exist_ok=True)
     a = os.path.join(outdir, f"{t['name']}_wave.hpp")
     with open(a, "w") as f:
@@ -297,4 +298,196 @@ def write(mech: Mechanism, p: dict, outdir: str) -> list:
     b = os.path.join(outdir, f"mech_{t['name']}.hip")
     with open(b, "w") as f:
         f.write(emit_unit(t))
-    return [a, b]
+    c = os.path.join(outdir, f"{t['name']}_wcode.hpp")
+    with open(c, "w") as f:
+        f.write(emit_code(t, ct))
+    return [a, b, c]
+
+
+# ---------------------------------------------------------------------------
+# Straight-line code for the pivot schedule and the triangular solves, and
+# fixed-trip-count term tables for F and W (the compiler can batch-issue every
+# table load and LDS access because no loop bound depends on data).
+# ---------------------------------------------------------------------------
+def _lane_test(mask: int, var: str = "lane") -> str:
+    """C condition selecting the lanes of a 64-bit mask."""
+    if mask == 0:
+        return "false"
+    if mask == (1 << 64) - 1:
+        return "true"
+    bits = [i for i in range(64) if (mask >> i) & 1]
+    if len(bits) == 1:
+        return f"({var} == {bits[0]})"
+    lo, hi = bits[0], bits[-1]
+    if hi - lo + 1 == len(bits):
+        return f"((unsigned)({var} - {lo}) < {hi - lo + 1}u)"
+    return f"((0x{mask:016x}ull >> {var}) & 1ull)"
+
+
+def _pack_chunks(items: list, tmax: int, width: int = 64):
+    """Split each (key, terms) item into chunks of <= tmax terms and place the
+    chunks into passes x lanes so that chunks of one key land in strictly
+    increasing passes.  Returns [pass][lane] -> (key, terms) or None."""
+    chunks = []
+    for key, terms in items:
+        cs = [terms[i:i + tmax] for i in range(0, len(terms), tmax)]
+        chunks.append((key, cs))
+    chunks.sort(key=lambda kc: -len(kc[1]))
+    passes: list[list] = []
+    for key, cs in chunks:
+        p = 0
+        for c in cs:
+            while True:
+                if p == len(passes):
+                    passes.append([])
+                if len(passes[p]) < width:
+                    passes[p].append((key, c))
+                    p += 1
+                    break
+                p += 1
+    return passes
+
+
+def build_code_tables(t: dict, tf: int = 4, tw: int = 2) -> dict:
+    nv = t["nv"]
+    coefs = t["coefs"]
+    # F: chunks of row terms (group index, coefficient)
+    fitems = []
+    for v in range(nv):
+        terms = [(t["fg"][k], coefs[t["fc"][k]]) for k in range(t["fofs"][v], t["fofs"][v + 1])]
+        if terms:
+            fitems.append((v, terms))
+    fpasses = _pack_chunks(fitems, tf)
+    # W: chunks of entry terms (partial index, coefficient) keyed by LU address
+    witems = []
+    for e, addr in enumerate(t["waddr"]):
+        terms = [(t["wpi"][k], coefs[t["wci"][k]]) for k in range(t["wofs"][e], t["wofs"][e + 1])]
+        if terms:
+            witems.append((addr, terms))
+    wpasses = _pack_chunks(witems, tw)
+    ZF = t["ng"]  # scratch slot holding 0.0 for padding (fluxes)
+    ZW = t["npart"]  # scratch slot holding 0.0 for padding (partials)
+
+    cidx = {c: i for i, c in enumerate(coefs)}
+    zero_c = cidx.get(0.0)
+    if zero_c is None:
+        zero_c = len(coefs)
+        coefs.append(0.0)
+
+    def lay(passes, tmax, zero, key_none):
+        # term = scratch index | coefficient index << 16, layout [pass][slot][lane]
+        term, key = [], []
+        for ps in passes:
+            for q in range(tmax):
+                for lane in range(64):
+                    if lane < len(ps) and q < len(ps[lane][1]):
+                        i, c = ps[lane][1][q]
+                        term.append(i | (cidx[c] << 16))
+                    else:
+                        term.append(zero | (zero_c << 16))
+            for lane in range(64):
+                key.append(ps[lane][0] if lane < len(ps) else key_none)
+        return term, key
+
+    fterm, frow = lay(fpasses, tf, ZF, 255)
+    wterm, waddr = lay(wpasses, tw, ZW, 0xFFFF)
+    return {"tf": tf, "tw": tw, "nfp": len(fpasses), "nwp": len(wpasses), "fterm": fterm, "frow": frow,
+            "wterm": wterm, "wkey": waddr}
+
+
+def emit_code(t: dict, ct: dict) -> str:
+    name = t["name"]
+    nv, nb = t["nv"], t["nb"]
+    R = t["R"]
+    Lset = set()
+    Uset = set()
+    for k in range(nv):
+        for i in t["lrow"][t["lofs"][k]:t["lofs"][k + 1]]:
+            Lset.add((i, k))
+        for j in t["ucol"][t["uofs"][k]:t["uofs"][k + 1]]:
+            Uset.add((k, j))
+    out = []
+    w = out.append
+    w(f"// Generated by mkpp_amd.gen.emit_wave: straight-line LU and solves for {name}.")
+    w("// Do not edit: regenerate with `python -m mkpp_amd.gen`.")
+    w("#pragma once")
+    w(f'#include "{name}_wave.hpp"')
+    w("")
+    w(f"namespace mkpp {{ namespace gen {{ namespace {name}_wave {{")
+    w("")
+    w(_arr("uint32_t", "CF_TERM", ct["fterm"]))
+    w(_arr("uint8_t", "CF_ROW", ct["frow"]))
+    w(_arr("uint32_t", "CW_TERM", ct["wterm"]))
+    w(_arr("uint16_t", "CW_ADDR", ct["wkey"]))
+    w("")
+    w("struct Code {")
+    w(f"  static constexpr int TF = {ct['tf']}, NFP = {ct['nfp']}, TW = {ct['tw']}, NWP = {ct['nwp']};")
+    w("")
+    # ---- factorization ----
+    w("  // Right-looking factorization: pivot k divides column k of every row in")
+    w("  // L(k) by U(k,k) and subtracts l * U(k, j) for j in U(k), each product in")
+    w("  // ascending-k order per entry (the reference's per-entry sequence).")
+    w("  static __device__ __forceinline__ void factor(double* __restrict__ lu, const int (&rb)[" + str(nb) + "], int lane) {")
+    for k in range(nv):
+        rows = t["lrow"][t["lofs"][k]:t["lofs"][k + 1]]
+        if not rows:
+            continue
+        cols = t["ucol"][t["uofs"][k]:t["uofs"][k + 1]]
+        w(f"    {{  // pivot {k}: {len(rows)} rows x {len(cols)} columns")
+        w(f"      const double d = lu[{R[k] + k}];")
+        for j in cols:
+            w(f"      const double u{j} = lu[{R[k] + j}];")
+        for b in range(nb):
+            m = 0
+            for i in rows:
+                if i // 64 == b:
+                    m |= 1 << (i % 64)
+            if not m:
+                continue
+            w(f"      if ({_lane_test(m)}) {{")
+            w(f"        double* r = lu + rb[{b}];")
+            w(f"        const double l = r[{k}] / d;")
+            w(f"        r[{k}] = l;")
+            for j in cols:
+                w(f"        r[{j}] = fma(-l, u{j}, r[{j}]);")
+            w("      }")
+        w("    }")
+    w("  }")
+    w("")
+    # ---- forward and backward substitution ----
+    w("  // Forward substitution, column oriented: x_k = w_k is final when column k")
+    w("  // is reached; rows of L(:, k) subtract L(i,k) x_k (ascending k per row).")
+    w("  static __device__ __forceinline__ void forward(const double* __restrict__ lu, const int (&rb)[" + str(nb) + "], double (&wv)[" + str(nb) + "], int lane) {")
+    for k in range(nv):
+        masks = [0] * nb
+        for (i, kk) in Lset:
+            if kk == k:
+                masks[i // 64] |= 1 << (i % 64)
+        if not any(masks):
+            continue
+        w(f"    {{ const double xk = readlane_d(wv[{k // 64}], {k % 64});")
+        for b in range(nb):
+            if masks[b]:
+                w(f"      if ({_lane_test(masks[b])}) wv[{b}] = fma(-lu[rb[{b}] + {k}], xk, wv[{b}]);")
+        w("    }")
+    w("  }")
+    w("")
+    w("  // Backward substitution, column oriented: x_k = w_k * (1 / U(k,k)), then rows")
+    w("  // of U(:, k) above the diagonal subtract U(i,k) x_k.")
+    w("  static __device__ __forceinline__ void backward(const double* __restrict__ lu, const int (&rb)[" + str(nb) + "], double (&wv)[" + str(nb) + "], const double (&rdiag)[" + str(nb) + "], double (&x)[" + str(nb) + "], int lane) {")
+    for k in reversed(range(nv)):
+        masks = [0] * nb
+        for (i, kk) in Uset:
+            if kk == k and i < k:
+                masks[i // 64] |= 1 << (i % 64)
+        w(f"    {{ const double xk = readlane_d(wv[{k // 64}] * rdiag[{k // 64}], {k % 64});")
+        w(f"      if (lane == {k % 64}) x[{k // 64}] = xk;")
+        for b in range(nb):
+            if masks[b]:
+                w(f"      if ({_lane_test(masks[b])}) wv[{b}] = fma(-lu[rb[{b}] + {k}], xk, wv[{b}]);")
+        w("    }")
+    w("  }")
+    w("};")
+    w("")
+    w(f"}} }} }}  // namespace mkpp::gen::{name}_wave")
+    return "\n".join(out) + "\n"
