@@ -175,8 +175,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
           acc = fma(T_::COEF[tm >> 16], scr[tm & 0xFFFFu], acc);
         }
         const int row = T_::CF_ROW[p * 64 + lane];
-        const int r = row == 255 ? NV : row;
-        facc[r] = facc[r] + acc;
+        if (row != 255) facc[row] = facc[row] + acc;
       }
 #pragma unroll
       for (int b = 0; b < NB; ++b) f[b] = valid[b] ? facc[b * 64 + lane] : 0.0;
@@ -188,23 +187,27 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
       for (int p = 0; p < NGP; ++p) {
         const int g = p * 64 + lane;
         if (g < NG) {
+          // all three reactant entries, absent ones with power 0 (ipow = 1.0):
+          // the products equal the oracle's term by term
           const int np = T_::GNPOW[g];
           const uint8_t* s = &T_::GPOWS[g * 3];
           const uint8_t* q = &T_::GPOWP[g * 3];
           const int off = T_::POFF[g];
-          for (int e = 0; e < np; ++e) {
-            // d flux / d y_e = k * p * y_e^(p-1) * prod_{other entries} y^p
-            double t = kreg[p] * (double)q[e];
-            double ip = 1.0;
-            for (int r = 1; r < q[e]; ++r) ip = (r == 1) ? ylds[s[e]] : ip * ylds[s[e]];
-            t = t * ip;
-            for (int o = 0; o < np; ++o) {
-              if (o == e) continue;
-              double po = ylds[s[o]];
-              for (int r = 1; r < q[o]; ++r) po = po * ylds[s[o]];
-              t = t * po;
-            }
-            scr[off + e] = t;
+          double yv[3], pw[3], pm1[3];
+#pragma unroll
+          for (int e = 0; e < 3; ++e) {
+            yv[e] = ylds[s[e]];
+            const double y2 = yv[e] * yv[e];
+            pw[e] = q[e] == 0 ? 1.0 : (q[e] == 1 ? yv[e] : (q[e] == 2 ? y2 : y2 * yv[e]));
+            pm1[e] = q[e] <= 1 ? 1.0 : (q[e] == 2 ? yv[e] : y2);
+          }
+#pragma unroll
+          for (int e = 0; e < 3; ++e) {
+            double t = (kreg[p] * (double)q[e]) * pm1[e];
+#pragma unroll
+            for (int o = 0; o < 3; ++o)
+              if (o != e) t = t * pw[o];
+            if (e < np) scr[off + e] = t;
           }
         }
       }
@@ -222,8 +225,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
           acc = fma(T_::COEF[tm >> 16], scr[tm & 0xFFFFu], acc);
         }
         const int ad = T_::CW_ADDR[p * 64 + lane];
-        const int adr = ad == 0xFFFF ? W::SPAN : ad;
-        lu[adr] = lu[adr] - acc;
+        if (ad != 0xFFFF) lu[ad] = lu[ad] - acc;
       }
     };
 
@@ -251,51 +253,68 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
         ++att_steps;
         dt = fmin(dt, dt_total - t);
         const double inv_g_dt = 1.0 / (Tab::G * dt);
+        // An opaque copy of the lane id: the generated code tests hundreds of
+        // constant lane masks, and with a visible loop-invariant lane id the
+        // compiler hoists every resulting exec mask out of the step loop and
+        // spills them.  Recomputing a mask next to its use is cheaper.
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
         double Sx[NB], K[S][NB], F[NB], rdiag[NB];
 #pragma unroll
         for (int b = 0; b < NB; ++b) Sx[b] = x[b];
         store_y(Sx);
         assemble_w(inv_g_dt);
-        C_::factor(lu, rb, lane);
+        C_::factor(lu, rb, ln);
 #pragma unroll
         for (int b = 0; b < NB; ++b) rdiag[b] = valid[b] ? 1.0 / lu[rb[b] + b * 64 + lane] : 0.0;
-#pragma unroll
+        // Stage loop kept rolled so that one copy of the F and solve code serves
+        // every stage.  Coefficients are read by the (uniform) stage index; a
+        // zero coefficient skips its term and a unit one adds K bare, exactly
+        // the textual form of the stage sums.
+#pragma unroll 1
         for (int s = 0; s < S; ++s) {
           double wv[NB];
           const int base = s * (s - 1) / 2;
-          if (s == 0) {
-            eval_rates(F);
-          } else if (Tab::NEWF[s]) {
-            double Y[NB];
+          if (s == 0 || Tab::NEWF[s]) {
+            if (s > 0) {
+              double Y[NB];
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-              double v = Sx[b];
+              for (int b = 0; b < NB; ++b) {
+                double v = Sx[b];
 #pragma unroll
-              for (int j = 0; j < s; ++j) {
-                const double c = Tab::A[base + j];
-                if (c == 0.0) continue;
-                v = (c == 1.0) ? v + K[j][b] : v + c * K[j][b];
+                for (int j = 0; j < S - 1; ++j) {
+                  const double c = j < s ? Tab::A[base + j] : 0.0;
+                  if (c == 0.0) continue;
+                  v = (c == 1.0) ? v + K[j][b] : v + c * K[j][b];
+                }
+                Y[b] = v;
               }
-              Y[b] = v;
+              store_y(Y);
             }
-            store_y(Y);
             eval_rates(F);
           }
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
             double v = F[b];
-            if (s > 0) {
 #pragma unroll
-              for (int j = 0; j < s; ++j) {
-                const double c = Tab::C[base + j];
-                if (c == 0.0) continue;
-                v = v + (c / dt) * K[j][b];
-              }
+            for (int j = 0; j < S - 1; ++j) {
+              const double c = j < s ? Tab::C[base + j] : 0.0;
+              if (c == 0.0) continue;
+              v = v + (c / dt) * K[j][b];
             }
             wv[b] = valid[b] ? v : 0.0;
           }
-          C_::forward(lu, rb, wv, lane);
-          C_::backward(lu, rb, wv, rdiag, K[s], lane);
+          int ls = ln;
+          asm volatile("" : "+v"(ls));
+          double xs[NB];
+          C_::forward(lu, rb, wv, ls);
+          C_::backward(lu, rb, wv, rdiag, xs, ls);
+#pragma unroll
+          for (int j = 0; j < S; ++j)
+            if (j == s) {
+#pragma unroll
+              for (int b = 0; b < NB; ++b) K[j][b] = xs[b];
+            }
         }
         // error norm: rows summed in logical order (fixed species add zeros)
         double term[NB];
