@@ -36,6 +36,9 @@ def main():
     ap.add_argument("--cells", type=int, default=262144)
     ap.add_argument("--cpu-sample", type=int, default=8192, help="cells timed on the host (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB elementwise copy (x * 1.0) three times: the known-size kernel "
+                         "that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): cells are independent, "
@@ -48,6 +51,15 @@ def main():
     from mkpp_amd import native, workloads
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        nbytes = 256 << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_kernel": "mul(x, 1.0) float64", "bytes_read": nbytes, "bytes_written": nbytes}))
+        return
     w = workloads.saprc99_urban(args.cells)
     plan = json.load(open(os.path.join(ROOT, "mkpp_amd", "plans", "saprc99.json")))
     step_flops = plan["step_flops"]["rodas3"]

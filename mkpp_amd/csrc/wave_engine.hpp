@@ -110,7 +110,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
     rb[b] = valid[b] ? (int)T_::R[row] : 0;
   }
 
-  for (long cell = blockIdx.x; cell < a.ncells; cell += gridDim.x) {
+  // XCD-aware cell order: workgroups go round-robin to the 8 XCDs, so block b
+  // runs on XCD b % 8.  Giving each XCD a contiguous range of cells lets the
+  // 16 cells that share a 128-byte line of every species row (conc is
+  // species-major) be served by one L2 instead of eight.  gridDim.x is a
+  // multiple of 8, which makes the map a bijection on [0, gridDim.x).
+  const long first = (long)(blockIdx.x % 8u) * (gridDim.x / 8u) + blockIdx.x / 8u;
+  for (long cell = first; cell < a.ncells; cell += gridDim.x) {
     // ---------------- load ----------------
     double x[NB], atol[NB], rtol[NB];
 #pragma unroll
@@ -483,7 +489,8 @@ template <class W>
 int launch_wave_integrate(const DeviceArgs& a, hipStream_t s) {
   if (a.ncells <= 0) return 0;
   const size_t lds = sizeof(double) * W::LDS_DOUBLES;
-  const unsigned grid = (unsigned)(a.ncells < (1L << 30) ? a.ncells : (1L << 30));
+  const long cells = a.ncells < (1L << 30) ? a.ncells : (1L << 30);
+  const unsigned grid = (unsigned)((cells + 7) / 8 * 8);  // multiple of 8: see the cell map in the kernel
   switch (a.method) {
     case ROS2: wave_integrate_kernel<W, TabRos2><<<grid, 64, lds, s>>>(a); break;
     case ROS3: wave_integrate_kernel<W, TabRos3><<<grid, 64, lds, s>>>(a); break;
