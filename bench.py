@@ -36,6 +36,8 @@ def main():
     ap.add_argument("--cells", type=int, default=262144)
     ap.add_argument("--cpu-sample", type=int, default=8192, help="cells timed on the host (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-pcie", dest="pcie", action="store_false",
+                    help="skip the host-buffer (handle API) run that is reported as pcie_inclusive")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB elementwise copy (x * 1.0) three times: the known-size kernel "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -109,6 +111,21 @@ def main():
     mean_kernel_s = float(np.mean(kernel_ms)) / 1e3
     achieved = flops_per_launch / mean_kernel_s
 
+    # The reference boundary hands over host buffers: the same cells through the
+    # handle API (stage in, integrate, copy back), reported beside `value`.
+    pcie = None
+    if args.pcie:
+        c_host = np.ascontiguousarray(w["conc"]).copy()
+        h = native.Handle("saprc99", ncell)
+        h.bind(c_host, w["temp"], w["pressure"], w["air_density"], w["rate_parameters"])
+        h.set_solver("rodas3", w["atol"], w["rtol"], w["seed"])
+        t2 = time.perf_counter()
+        h.integrate(w["dt"], 1)
+        pcie_s = time.perf_counter() - t2
+        h.close()
+        pcie = {"value": ncell / pcie_s, "unit": "grid-cells/s", "ms": pcie_s * 1e3,
+                "what": "mkpp_integrate on host LayoutLeft buffers: host-to-device staging, kernel, copy back"}
+
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "saprc99_rodas3_262144_traffic.json")
     if os.path.exists(tfile):
@@ -154,6 +171,7 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": traffic,
                      "flops_per_attempted_step": step_flops},
         "cpu_baseline": cpu,
+        "pcie_inclusive": pcie,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
