@@ -36,6 +36,8 @@ def main():
     ap.add_argument("--cells", type=int, default=262144)
     ap.add_argument("--cpu-sample", type=int, default=8192, help="cells timed on the host (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-batch", dest="batch", action="store_false",
+                    help="skip the mixed-mechanism persistent-vs-plain comparison reported as batching")
     ap.add_argument("--no-pcie", dest="pcie", action="store_false",
                     help="skip the host-buffer (handle API) run that is reported as pcie_inclusive")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -126,6 +128,8 @@ def main():
         pcie = {"value": ncell / pcie_s, "unit": "grid-cells/s", "ms": pcie_s * 1e3,
                 "what": "mkpp_integrate on host LayoutLeft buffers: host-to-device staging, kernel, copy back"}
 
+    batching = run_batching(np) if args.batch else None
+
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "saprc99_rodas3_262144_traffic.json")
     if os.path.exists(tfile):
@@ -172,10 +176,40 @@ def main():
                      "flops_per_attempted_step": step_flops},
         "cpu_baseline": cpu,
         "pcie_inclusive": pcie,
+        "batching": batching,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
     print(json.dumps(out))
+
+
+def run_batching(np):
+    """131072 saprc99 + 65536 carbon + 65536 small_strato cells (Rodas3, rtol
+    1e-3, 1 h): one persistent launch (autotuned) against one plain launch per
+    mechanism, same inputs, fresh state per timed run."""
+    from mkpp_amd import batch, workloads
+
+    run = batch.BatchRun(workloads.batch_mixed())
+    run.persistent()  # warm-up of both paths
+    run.plain()
+    tune = batch.autotune(run)
+    best = tune["best"]
+    persistent_ms = run.timed(lambda: run.persistent(best["cells_per_lane"], best["workgroups"]))
+    acc = run.accepted()
+    bad = int((run.statuses() != 0).sum())
+    plain_ms = run.timed(run.plain)
+    allacc = np.concatenate(list(acc.values()))
+    edges = [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 1 << 30]
+    hist = {f"{edges[i]}-{edges[i + 1] - 1}": int(((allacc >= edges[i]) & (allacc < edges[i + 1])).sum())
+            for i in range(len(edges) - 1)}
+    hist = {k: v for k, v in hist.items() if v}
+    return {"workload": "131072 saprc99 (NOx 1e8-1e13, half night) + 65536 carbon + 65536 small_strato (half night), "
+                        "Rodas3 rtol 1e-3 atol 1e3, 1 h",
+            "persistent_ms": persistent_ms, "plain_ms": plain_ms, "persistent_over_plain": plain_ms / persistent_ms,
+            "tuned": best, "tuning_table": tune["table"], "nonzero_status_cells": bad,
+            "accepted_steps": {"min": int(allacc.min()), "max": int(allacc.max()),
+                               "spread": float(allacc.max() / max(allacc.min(), 1)), "histogram": hist,
+                               "mean_per_mechanism": {k: float(v.mean()) for k, v in acc.items()}}}
 
 
 if __name__ == "__main__":

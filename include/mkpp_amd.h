@@ -107,6 +107,44 @@ int mkpp_amd_device_evaluate(const char* mechanism, int what, const double* conc
                              const double* temp, const double* air_density, const double* rate_parameters,
                              long parameter_stride, double* out, long out_stride, void* stream);
 
+/* ---- 4. persistent batch over several mechanisms (replaces one launch per
+ * mechanism: src/mkpp_host/include/mkpp_host/dispatcher.hpp:258-362 and
+ * MechanismRegistry::execute, mechanism_manager.hpp:118-209) ----
+ * One launch integrates every segment's cells; wavefronts take work items
+ * (one cell of a wave-per-cell mechanism, 64*cells_per_lane cells of a
+ * lane-per-cell one) from eight device queues, one per XCD, with stealing.
+ * Segments may appear in any order; pointers and layouts are those of
+ * mkpp_amd_device_integrate.  `workspace` is device memory of at least
+ * MKPP_AMD_BATCH_WORKSPACE_BYTES, private to this call until it completes
+ * (the queue heads; the call resets them on the stream, so it may be captured
+ * in a graph).  workgroups = 0 launches as many as the device holds at once.
+ * Allocates nothing. */
+#define MKPP_AMD_BATCH_MAX_SEGMENTS 8
+#define MKPP_AMD_BATCH_WORKSPACE_BYTES 32
+
+typedef struct mkpp_amd_batch_segment {
+  const char* mechanism;
+  double* conc;
+  long num_cells;
+  long stride;
+  const double* temp;
+  const double* air_density;
+  const double* rate_parameters;
+  long parameter_stride;
+  const double* atol;
+  const double* rtol;
+  int* status;
+  int* accepted;
+  int* attempted;
+} mkpp_amd_batch_segment;
+
+int mkpp_amd_device_integrate_batch(const mkpp_amd_batch_segment* segments, int num_segments, double dt, int steps,
+                                    int method, double initial_step_seed, long max_attempts, int cells_per_lane,
+                                    int workgroups, void* workspace, void* stream);
+/* Workgroups of the batch kernel for `method` that one CU holds at once, and
+ * the CU count of the current device (the default grid is their product). */
+int mkpp_amd_batch_capacity(int method, int* per_cu, int* num_cus);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,7 @@ HIP_FLAGS = [
 
 def sources() -> list[str]:
     srcs = sorted(glob.glob(os.path.join(CSRC, "gen", "mech_*.hip")))
+    srcs += sorted(glob.glob(os.path.join(CSRC, "gen", "batch_*.hip")))
     srcs += sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     srcs.append(os.path.join(CSRC, "api.cpp"))
     return srcs

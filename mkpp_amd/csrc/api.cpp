@@ -10,6 +10,7 @@
 // device-pointer entry points skip the staging.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -18,6 +19,7 @@
 #include <vector>
 
 #include "../../include/mkpp_amd.h"
+#include "batch.hpp"
 #include "runtime.hpp"
 
 namespace mkpp {
@@ -526,3 +528,109 @@ int mkpp_amd_device_evaluate(const char* mechanism, int what, const double* conc
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Persistent batch (mkpp_amd.h section 4)
+// ---------------------------------------------------------------------------
+namespace mkpp {
+struct BatchEngineInfo {
+  const char* name;
+  int wave;  // 1: one cell per wavefront, 0: one cell per lane
+  int lds_doubles;
+};
+#include "gen/batch_list.inc"  // kBatchEngines, kBatchLaunch
+
+static int batch_engine_index(const char* name) {
+  for (int i = 0; i < (int)(sizeof(kBatchEngines) / sizeof(kBatchEngines[0])); ++i)
+    if (std::strcmp(kBatchEngines[i].name, name) == 0) return i;
+  return -1;
+}
+}  // namespace mkpp
+
+int mkpp_amd_batch_capacity(int method, int* per_cu, int* num_cus) {
+  if (!per_cu || !num_cus) return MKPP_ERROR_NULL_POINTER;
+  if (method < mkpp::ROS2 || method > mkpp::RODAS4) return MKPP_ERROR_INVALID_ARGUMENT;
+  size_t lds = 0;
+  for (const auto& e : mkpp::kBatchEngines) lds = std::max(lds, sizeof(double) * (size_t)e.lds_doubles);
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return MKPP_ERROR_INVALID_STATE;
+  *per_cu = mkpp::kBatchLaunch[method]()->resident_per_cu(lds);
+  *num_cus = cus;
+  return *per_cu > 0 ? MKPP_SUCCESS : MKPP_ERROR_INVALID_STATE;
+}
+
+int mkpp_amd_device_integrate_batch(const mkpp_amd_batch_segment* segments, int num_segments, double dt, int steps,
+                                    int method, double initial_step_seed, long max_attempts, int cells_per_lane,
+                                    int workgroups, void* workspace, void* stream) {
+  if (!segments || !workspace) return MKPP_ERROR_NULL_POINTER;
+  if (num_segments < 1 || num_segments > mkpp::BATCH_MAX_SEGMENTS || !std::isfinite(dt) || dt <= 0.0 || steps < 1 ||
+      method < mkpp::ROS2 || method > mkpp::RODAS4 || !(initial_step_seed > 0.0) || cells_per_lane < 1 ||
+      cells_per_lane > 64 || workgroups < 0)
+    return MKPP_ERROR_INVALID_ARGUMENT;
+  mkpp::BatchArgs b{};
+  size_t lds = 0;
+  // wave-per-cell segments first: their items are the longest, so they start
+  // before the cheap lane items that fill the tail
+  int order[mkpp::BATCH_MAX_SEGMENTS], n = 0;
+  for (int pass = 1; pass >= 0; --pass)
+    for (int i = 0; i < num_segments; ++i) {
+      const mkpp_amd_batch_segment& g = segments[i];
+      if (!g.mechanism || !g.conc || !g.atol || !g.rtol) return MKPP_ERROR_NULL_POINTER;
+      const int e = mkpp::batch_engine_index(g.mechanism);
+      if (e < 0) return MKPP_ERROR_NOT_FOUND;
+      if (g.num_cells < 0 || g.stride < g.num_cells || (g.rate_parameters && g.parameter_stride < g.num_cells))
+        return MKPP_ERROR_INVALID_ARGUMENT;
+      if (mkpp::kBatchEngines[e].wave == pass) order[n++] = i;
+    }
+  long per_queue_max = 0;
+  for (int k = 0; k < num_segments; ++k) {
+    const mkpp_amd_batch_segment& g = segments[order[k]];
+    const int e = mkpp::batch_engine_index(g.mechanism);
+    mkpp::BatchSegment& s = b.seg[k];
+    DeviceArgs& a = s.args;
+    a.conc = g.conc;
+    a.ncells = g.num_cells;
+    a.stride = g.stride;
+    a.temp = g.temp;
+    a.air_density = g.air_density;
+    a.jvals = g.rate_parameters;
+    a.jstride = g.parameter_stride;
+    a.dt = dt;
+    a.steps = steps;
+    a.method = method;
+    a.seed = initial_step_seed;
+    a.atol = g.atol;
+    a.rtol = g.rtol;
+    a.status = g.status;
+    a.accepted = g.accepted;
+    a.attempted = g.attempted;
+    a.max_attempts = max_attempts;
+    s.engine = e;
+    if (mkpp::kBatchEngines[e].wave) {
+      s.lanes = 0;
+      s.items = g.num_cells;
+      lds = std::max(lds, sizeof(double) * (size_t)mkpp::kBatchEngines[e].lds_doubles);
+    } else {
+      s.lanes = cells_per_lane;
+      const long per_item = 64L * cells_per_lane;
+      s.items = (g.num_cells + per_item - 1) / per_item;
+    }
+    per_queue_max += s.items / mkpp::BATCH_QUEUES + 1;
+  }
+  if (per_queue_max >= (1L << 31)) return MKPP_ERROR_INVALID_ARGUMENT;  // 32-bit queue heads
+  b.nseg = num_segments;
+  b.heads = static_cast<unsigned int*>(workspace);
+  const mkpp::BatchLaunch* L = mkpp::kBatchLaunch[method]();
+  if (workgroups == 0) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return MKPP_ERROR_INVALID_STATE;
+    const int per_cu = L->resident_per_cu(lds);
+    if (per_cu <= 0) return MKPP_ERROR_INVALID_STATE;
+    workgroups = per_cu * cus;
+  }
+  return L->launch(b, lds, workgroups, (hipStream_t)stream) == 0 ? MKPP_SUCCESS : MKPP_ERROR_INVALID_STATE;
+}

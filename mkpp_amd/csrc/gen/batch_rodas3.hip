@@ -1,0 +1,4 @@
+// Generated by mkpp_amd.gen.emit_batch: the persistent batch kernel for Rodas3.
+#include "batch_engines.hpp"
+
+MKPP_BATCH_METHOD(TabRodas3)

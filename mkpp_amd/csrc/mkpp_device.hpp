@@ -149,7 +149,7 @@ __device__ __forceinline__ double step_factor(double err) {
 template <class M, class Tab>
 __device__ __forceinline__ int integrate_cell(double (&x)[M::NV], const double (&yf)[M::NF], const double* k,
                                               const double* dk, double dt_total, double seed,
-                                              const double (&atol)[M::NV], const double (&rtol)[M::NV],
+                                              const double* __restrict__ atol, const double* __restrict__ rtol,
                                               long max_attempts, int& accepted, int& attempted) {
   constexpr int NV = M::NV, S = Tab::S;
   const double g = Tab::G;
@@ -228,7 +228,8 @@ __device__ __forceinline__ int integrate_cell(double (&x)[M::NV], const double (
         first = false;
       }
       const double ymax = fmax(fabs(x[i]), fabs(ynew));
-      const double sci = atol[i] + rtol[i] * ymax;
+      // per-species tolerances are read where used (uniform loads), not held in registers
+      const double sci = atol[M::var_phys(i)] + rtol[M::var_phys(i)] * ymax;
       err_sq += (yerr / sci) * (yerr / sci);
     }
     double err = sqrt(err_sq / M::N);
@@ -272,17 +273,14 @@ __device__ __forceinline__ bool all_finite(const double* v, int n) {
 }
 
 // One lane per cell: load, hoist rate constants, integrate `steps` times, store.
+// Shared by the plain launch below and the persistent batch kernel.
 template <class M, class Tab>
-__global__ void __launch_bounds__(128) integrate_lane_kernel(DeviceArgs a) {
-  const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cell >= a.ncells) return;
-  double x[M::NV], yf[M::NF], jv[M::NPHOTO], atol[M::NV], rtol[M::NV];
+__device__ __forceinline__ void integrate_lane_cell(const DeviceArgs& a, long cell) {
+  double x[M::NV], yf[M::NF], jv[M::NPHOTO];
   double k[M::NG], dk[M::NG];
 #pragma unroll
   for (int v = 0; v < M::NV; ++v) {
     x[v] = a.conc[cell + a.stride * M::var_phys(v)];
-    atol[v] = a.atol[M::var_phys(v)];
-    rtol[v] = a.rtol[M::var_phys(v)];
   }
 #pragma unroll
   for (int q = 0; q < M::NF; ++q) yf[q] = (q < M::NFIXED) ? a.conc[cell + a.stride * M::fixed_phys(q)] : 0.0;
@@ -304,13 +302,19 @@ __global__ void __launch_bounds__(128) integrate_lane_kernel(DeviceArgs a) {
   M::rate_constants(T, Mtb, jv, k, dk);
   int acc = 0, att = 0;
   for (int step = 0; step < a.steps && status == SUCCESS; ++step)
-    status = integrate_cell<M, Tab>(x, yf, k, dk, a.dt, a.seed, atol, rtol, a.max_attempts, acc, att);
+    status = integrate_cell<M, Tab>(x, yf, k, dk, a.dt, a.seed, a.atol, a.rtol, a.max_attempts, acc, att);
 #pragma unroll
   for (int v = 0; v < M::NV; ++v) a.conc[cell + a.stride * M::var_phys(v)] = x[v];
   if (status == SUCCESS && !all_finite(x, M::NV)) status = ERR_NUMERICAL_FAILURE;
   if (a.status) a.status[cell] = status;
   if (a.accepted) a.accepted[cell] += acc;
   if (a.attempted) a.attempted[cell] += att;
+}
+
+template <class M, class Tab>
+__global__ void __launch_bounds__(128) integrate_lane_kernel(DeviceArgs a) {
+  const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell < a.ncells) integrate_lane_cell<M, Tab>(a, cell);
 }
 
 // Evaluation kernels (mkpp_evaluate_rates / mkpp_evaluate_jacobian).

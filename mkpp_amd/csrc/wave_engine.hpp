@@ -90,16 +90,14 @@ __device__ __forceinline__ double bank(const double (&v)[NB], int row) {
 //   facc [NV + 1]    F row accumulators; slot NV absorbs padding
 // ---------------------------------------------------------------------------
 template <class W, class Tab>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) wave_integrate_kernel(DeviceArgs a) {
+__device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long cell, double* lds, int lane) {
   using T_ = WaveTables<W>;
   using C_ = typename T_::Code;
   constexpr int NV = W::NV, NB = W::NB, NG = W::NG, NGP = (NG + 63) / 64, S = Tab::S;
-  extern __shared__ double lds[];
   double* lu = lds;
   double* ylds = lu + (W::SPAN + 1);
   double* scr = ylds + W::NY;
   double* facc = scr + W::NSCR;
-  const int lane = threadIdx.x;
 
   int rb[NB];
   bool valid[NB];
@@ -110,290 +108,295 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
     rb[b] = valid[b] ? (int)T_::R[row] : 0;
   }
 
+  // ---------------- load ----------------
+  double x[NB], atol[NB], rtol[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int row = b * 64 + lane;
+    const int ph = valid[b] ? (int)T_::VARPHYS[row] : 0;
+    x[b] = valid[b] ? a.conc[cell + a.stride * ph] : 0.0;
+    atol[b] = valid[b] ? a.atol[ph] : 1.0;
+    rtol[b] = valid[b] ? a.rtol[ph] : 0.0;
+  }
+  if (lane < W::NF) ylds[NV + lane] = a.conc[cell + a.stride * (long)T_::FIXPHYS[lane]];
+  if (lane == 0) ylds[W::ONE] = 1.0;
+  const double Tk = a.temp ? a.temp[cell] : 300.0;
+  const double rho = a.air_density ? a.air_density[cell] : 2.4476e19;
+  const double Mtb = (W::THIRD_BODY >= 0) ? a.conc[cell + a.stride * (long)(W::THIRD_BODY >= 0 ? W::THIRD_BODY : 0)] : rho;
+
+  // non-finite pre-check (dispatcher.hpp:289-307) when a status array is given
+  if (a.status != nullptr) {
+    bool ok = true;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) ok = ok && isfinite(x[b]);
+    if (lane < W::NF) ok = ok && isfinite(ylds[NV + lane]);
+    if (__ballot(!ok) != 0ull) {
+      if (lane == 0) a.status[cell] = ERR_NON_FINITE_STATE;
+      return;
+    }
+  }
+
+  // ---------------- rate constants (once per cell and call) ----------------
+  double kreg[NGP];
+#pragma unroll
+  for (int p = 0; p < NGP; ++p) {
+    const int g = p * 64 + lane;
+    kreg[p] = 0.0;
+    if (g < NG) {
+      const int law = T_::LAW[g];
+      const int ph = T_::PHOTO[g];
+      const double jv = (law == 0 && a.jvals != nullptr) ? a.jvals[cell + a.jstride * ph] : 0.0;
+      kreg[p] = wave_rate(law, &T_::PARAMS[g * 7], Tk, Mtb, jv);
+    }
+  }
+
+  // F at the state in ylds: fluxes by group lanes, then fixed-size term
+  // chunks accumulated per row (chunks of one row sit in successive passes)
+  auto eval_rates = [&](double (&f)[NB]) {
+#pragma unroll
+    for (int p = 0; p < NGP; ++p) {
+      const int g = p * 64 + lane;
+      if (g < NG) {
+        const uint8_t* fa = &T_::GFAC[g * 3];
+        scr[g] = kreg[p] * (((1.0 * ylds[fa[0]]) * ylds[fa[1]]) * ylds[fa[2]]);
+      }
+    }
+    if (lane == 0) scr[NG] = 0.0;
+    for (int r = lane; r <= NV; r += 64) facc[r] = 0.0;
+#pragma unroll
+    for (int p = 0; p < C_::NFP; ++p) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < C_::TF; ++q) {
+        const uint32_t tm = T_::CF_TERM[(p * C_::TF + q) * 64 + lane];
+        acc = fma(T_::COEF[tm >> 16], scr[tm & 0xFFFFu], acc);
+      }
+      const int row = T_::CF_ROW[p * 64 + lane];
+      if (row != 255) facc[row] = facc[row] + acc;
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) f[b] = valid[b] ? facc[b * 64 + lane] : 0.0;
+  };
+
+  // W = I/(g h) - J(y) into the LU storage
+  auto assemble_w = [&](double inv_g_dt) {
+#pragma unroll
+    for (int p = 0; p < NGP; ++p) {
+      const int g = p * 64 + lane;
+      if (g < NG) {
+        // all three reactant entries, absent ones with power 0 (ipow = 1.0):
+        // the products equal the oracle's term by term
+        const int np = T_::GNPOW[g];
+        const uint8_t* s = &T_::GPOWS[g * 3];
+        const uint8_t* q = &T_::GPOWP[g * 3];
+        const int off = T_::POFF[g];
+        double yv[3], pw[3], pm1[3];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          yv[e] = ylds[s[e]];
+          const double y2 = yv[e] * yv[e];
+          pw[e] = q[e] == 0 ? 1.0 : (q[e] == 1 ? yv[e] : (q[e] == 2 ? y2 : y2 * yv[e]));
+          pm1[e] = q[e] <= 1 ? 1.0 : (q[e] == 2 ? yv[e] : y2);
+        }
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          double t = (kreg[p] * (double)q[e]) * pm1[e];
+#pragma unroll
+          for (int o = 0; o < 3; ++o)
+            if (o != e) t = t * pw[o];
+          if (e < np) scr[off + e] = t;
+        }
+      }
+    }
+    if (lane == 0) scr[W::NPART] = 0.0;
+    for (int i = lane; i <= W::SPAN; i += 64) lu[i] = 0.0;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      if (valid[b]) lu[rb[b] + b * 64 + lane] = inv_g_dt;
+#pragma unroll
+    for (int p = 0; p < C_::NWP; ++p) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < C_::TW; ++q) {
+        const uint32_t tm = T_::CW_TERM[(p * C_::TW + q) * 64 + lane];
+        acc = fma(T_::COEF[tm >> 16], scr[tm & 0xFFFFu], acc);
+      }
+      const int ad = T_::CW_ADDR[p * 64 + lane];
+      if (ad != 0xFFFF) lu[ad] = lu[ad] - acc;
+    }
+  };
+
+  auto store_y = [&](const double (&y)[NB]) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      if (valid[b]) ylds[b * 64 + lane] = y[b];
+  };
+
+  // ---------------- integrate `steps` times over dt ----------------
+  int status = SUCCESS;
+  int acc_steps = 0, att_steps = 0;
+  for (int call = 0; call < a.steps && status == SUCCESS; ++call) {
+    const double dt_total = a.dt;
+    const double h_min = dt_total * 1.0e-15;
+    double t = 0.0, dt = dt_total * a.seed;
+    bool reject_last = false, reject_more = false;
+    long attempts = 0;
+    while (t < dt_total) {
+      if (a.max_attempts > 0 && attempts >= a.max_attempts) {
+        status = ERR_CONVERGENCE_FAILURE;
+        break;
+      }
+      ++attempts;
+      ++att_steps;
+      dt = fmin(dt, dt_total - t);
+      const double inv_g_dt = 1.0 / (Tab::G * dt);
+      // An opaque copy of the lane id: the generated code tests hundreds of
+      // constant lane masks, and with a visible loop-invariant lane id the
+      // compiler hoists every resulting exec mask out of the step loop and
+      // spills them.  Recomputing a mask next to its use is cheaper.
+      int ln = lane;
+      asm volatile("" : "+v"(ln));
+      double Sx[NB], K[S][NB], F[NB], rdiag[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) Sx[b] = x[b];
+      store_y(Sx);
+      assemble_w(inv_g_dt);
+      C_::factor(lu, rb, ln);
+#pragma unroll
+      for (int b = 0; b < NB; ++b) rdiag[b] = valid[b] ? 1.0 / lu[rb[b] + b * 64 + lane] : 0.0;
+      // Stage loop kept rolled so that one copy of the F and solve code serves
+      // every stage.  Coefficients are read by the (uniform) stage index; a
+      // zero coefficient skips its term and a unit one adds K bare, exactly
+      // the textual form of the stage sums.
+#pragma unroll 1
+      for (int s = 0; s < S; ++s) {
+        double wv[NB];
+        const int base = s * (s - 1) / 2;
+        if (s == 0 || Tab::NEWF[s]) {
+          if (s > 0) {
+            double Y[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+              double v = Sx[b];
+#pragma unroll
+              for (int j = 0; j < S - 1; ++j) {
+                const double c = j < s ? Tab::A[base + j] : 0.0;
+                if (c == 0.0) continue;
+                v = (c == 1.0) ? v + K[j][b] : v + c * K[j][b];
+              }
+              Y[b] = v;
+            }
+            store_y(Y);
+          }
+          eval_rates(F);
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          double v = F[b];
+#pragma unroll
+          for (int j = 0; j < S - 1; ++j) {
+            const double c = j < s ? Tab::C[base + j] : 0.0;
+            if (c == 0.0) continue;
+            v = v + (c / dt) * K[j][b];
+          }
+          wv[b] = valid[b] ? v : 0.0;
+        }
+        int ls = ln;
+        asm volatile("" : "+v"(ls));
+        double xs[NB];
+        C_::forward(lu, rb, wv, ls);
+        C_::backward(lu, rb, wv, rdiag, xs, ls);
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+          if (j == s) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) K[j][b] = xs[b];
+          }
+      }
+      // error norm: rows summed in logical order (fixed species add zeros)
+      double term[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        double ynew = Sx[b], yerr = 0.0;
+        bool first = true;
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+          const double m = Tab::M[j];
+          if (m == 0.0) continue;
+          ynew = ynew + ((m == 1.0) ? K[j][b] : m * K[j][b]);
+        }
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+          const double e = Tab::E[j];
+          if (e == 0.0) continue;
+          const double tt = (e == 1.0) ? K[j][b] : e * K[j][b];
+          yerr = first ? tt : yerr + tt;
+          first = false;
+        }
+        const double sci = atol[b] + rtol[b] * fmax(fabs(x[b]), fabs(ynew));
+        term[b] = valid[b] ? (yerr / sci) * (yerr / sci) : 0.0;
+      }
+      double err_sq = 0.0;
+      for (int i = 0; i < NV; ++i) err_sq += readlane_d(bank<NB>(term, i), i & 63);
+      double err = sqrt(err_sq / W::N);
+      err = fmax(err, 1.0e-10);
+      const double factor_ = step_factor<Tab>(err);
+      if (err <= 1.0) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          double msum = 0.0;
+          bool first = true;
+#pragma unroll
+          for (int j = 0; j < S; ++j) {
+            const double m = Tab::M[j];
+            if (m == 0.0) continue;
+            const double tt = (m == 1.0) ? K[j][b] : m * K[j][b];
+            msum = first ? tt : msum + tt;
+            first = false;
+          }
+          x[b] += msum;
+        }
+        t += dt;
+        ++acc_steps;
+        double next_dt = fmax(h_min, fmin(dt * factor_, dt_total));
+        if (reject_last) next_dt = fmin(next_dt, dt);
+        reject_last = false;
+        reject_more = false;
+        dt = next_dt;
+      } else {
+        dt *= reject_more ? 0.1 : factor_;
+        reject_more = reject_last;
+        reject_last = true;
+      }
+    }
+  }
+  // ---------------- store ----------------
+  bool fin = true;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if (valid[b]) a.conc[cell + a.stride * (long)T_::VARPHYS[b * 64 + lane]] = x[b];
+    fin = fin && (!valid[b] || isfinite(x[b]));
+  }
+  const bool all_fin = __ballot(!fin) == 0ull;
+  if (lane == 0) {
+    if (status == SUCCESS && !all_fin) status = ERR_NUMERICAL_FAILURE;
+    if (a.status) a.status[cell] = status;
+    if (a.accepted) a.accepted[cell] += acc_steps;
+    if (a.attempted) a.attempted[cell] += att_steps;
+  }
+}
+
+// One cell per wavefront over a grid-stride loop.
+template <class W, class Tab>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) wave_integrate_kernel(DeviceArgs a) {
+  extern __shared__ double lds[];
   // XCD-aware cell order: workgroups go round-robin to the 8 XCDs, so block b
   // runs on XCD b % 8.  Giving each XCD a contiguous range of cells lets the
   // 16 cells that share a 128-byte line of every species row (conc is
   // species-major) be served by one L2 instead of eight.  gridDim.x is a
   // multiple of 8, which makes the map a bijection on [0, gridDim.x).
   const long first = (long)(blockIdx.x % 8u) * (gridDim.x / 8u) + blockIdx.x / 8u;
-  for (long cell = first; cell < a.ncells; cell += gridDim.x) {
-    // ---------------- load ----------------
-    double x[NB], atol[NB], rtol[NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      const int row = b * 64 + lane;
-      const int ph = valid[b] ? (int)T_::VARPHYS[row] : 0;
-      x[b] = valid[b] ? a.conc[cell + a.stride * ph] : 0.0;
-      atol[b] = valid[b] ? a.atol[ph] : 1.0;
-      rtol[b] = valid[b] ? a.rtol[ph] : 0.0;
-    }
-    if (lane < W::NF) ylds[NV + lane] = a.conc[cell + a.stride * (long)T_::FIXPHYS[lane]];
-    if (lane == 0) ylds[W::ONE] = 1.0;
-    const double Tk = a.temp ? a.temp[cell] : 300.0;
-    const double rho = a.air_density ? a.air_density[cell] : 2.4476e19;
-    const double Mtb = (W::THIRD_BODY >= 0) ? a.conc[cell + a.stride * (long)(W::THIRD_BODY >= 0 ? W::THIRD_BODY : 0)] : rho;
-
-    // non-finite pre-check (dispatcher.hpp:289-307) when a status array is given
-    if (a.status != nullptr) {
-      bool ok = true;
-#pragma unroll
-      for (int b = 0; b < NB; ++b) ok = ok && isfinite(x[b]);
-      if (lane < W::NF) ok = ok && isfinite(ylds[NV + lane]);
-      if (__ballot(!ok) != 0ull) {
-        if (lane == 0) a.status[cell] = ERR_NON_FINITE_STATE;
-        continue;
-      }
-    }
-
-    // ---------------- rate constants (once per cell and call) ----------------
-    double kreg[NGP];
-#pragma unroll
-    for (int p = 0; p < NGP; ++p) {
-      const int g = p * 64 + lane;
-      kreg[p] = 0.0;
-      if (g < NG) {
-        const int law = T_::LAW[g];
-        const int ph = T_::PHOTO[g];
-        const double jv = (law == 0 && a.jvals != nullptr) ? a.jvals[cell + a.jstride * ph] : 0.0;
-        kreg[p] = wave_rate(law, &T_::PARAMS[g * 7], Tk, Mtb, jv);
-      }
-    }
-
-    // F at the state in ylds: fluxes by group lanes, then fixed-size term
-    // chunks accumulated per row (chunks of one row sit in successive passes)
-    auto eval_rates = [&](double (&f)[NB]) {
-#pragma unroll
-      for (int p = 0; p < NGP; ++p) {
-        const int g = p * 64 + lane;
-        if (g < NG) {
-          const uint8_t* fa = &T_::GFAC[g * 3];
-          scr[g] = kreg[p] * (((1.0 * ylds[fa[0]]) * ylds[fa[1]]) * ylds[fa[2]]);
-        }
-      }
-      if (lane == 0) scr[NG] = 0.0;
-      for (int r = lane; r <= NV; r += 64) facc[r] = 0.0;
-#pragma unroll
-      for (int p = 0; p < C_::NFP; ++p) {
-        double acc = 0.0;
-#pragma unroll
-        for (int q = 0; q < C_::TF; ++q) {
-          const uint32_t tm = T_::CF_TERM[(p * C_::TF + q) * 64 + lane];
-          acc = fma(T_::COEF[tm >> 16], scr[tm & 0xFFFFu], acc);
-        }
-        const int row = T_::CF_ROW[p * 64 + lane];
-        if (row != 255) facc[row] = facc[row] + acc;
-      }
-#pragma unroll
-      for (int b = 0; b < NB; ++b) f[b] = valid[b] ? facc[b * 64 + lane] : 0.0;
-    };
-
-    // W = I/(g h) - J(y) into the LU storage
-    auto assemble_w = [&](double inv_g_dt) {
-#pragma unroll
-      for (int p = 0; p < NGP; ++p) {
-        const int g = p * 64 + lane;
-        if (g < NG) {
-          // all three reactant entries, absent ones with power 0 (ipow = 1.0):
-          // the products equal the oracle's term by term
-          const int np = T_::GNPOW[g];
-          const uint8_t* s = &T_::GPOWS[g * 3];
-          const uint8_t* q = &T_::GPOWP[g * 3];
-          const int off = T_::POFF[g];
-          double yv[3], pw[3], pm1[3];
-#pragma unroll
-          for (int e = 0; e < 3; ++e) {
-            yv[e] = ylds[s[e]];
-            const double y2 = yv[e] * yv[e];
-            pw[e] = q[e] == 0 ? 1.0 : (q[e] == 1 ? yv[e] : (q[e] == 2 ? y2 : y2 * yv[e]));
-            pm1[e] = q[e] <= 1 ? 1.0 : (q[e] == 2 ? yv[e] : y2);
-          }
-#pragma unroll
-          for (int e = 0; e < 3; ++e) {
-            double t = (kreg[p] * (double)q[e]) * pm1[e];
-#pragma unroll
-            for (int o = 0; o < 3; ++o)
-              if (o != e) t = t * pw[o];
-            if (e < np) scr[off + e] = t;
-          }
-        }
-      }
-      if (lane == 0) scr[W::NPART] = 0.0;
-      for (int i = lane; i <= W::SPAN; i += 64) lu[i] = 0.0;
-#pragma unroll
-      for (int b = 0; b < NB; ++b)
-        if (valid[b]) lu[rb[b] + b * 64 + lane] = inv_g_dt;
-#pragma unroll
-      for (int p = 0; p < C_::NWP; ++p) {
-        double acc = 0.0;
-#pragma unroll
-        for (int q = 0; q < C_::TW; ++q) {
-          const uint32_t tm = T_::CW_TERM[(p * C_::TW + q) * 64 + lane];
-          acc = fma(T_::COEF[tm >> 16], scr[tm & 0xFFFFu], acc);
-        }
-        const int ad = T_::CW_ADDR[p * 64 + lane];
-        if (ad != 0xFFFF) lu[ad] = lu[ad] - acc;
-      }
-    };
-
-    auto store_y = [&](const double (&y)[NB]) {
-#pragma unroll
-      for (int b = 0; b < NB; ++b)
-        if (valid[b]) ylds[b * 64 + lane] = y[b];
-    };
-
-    // ---------------- integrate `steps` times over dt ----------------
-    int status = SUCCESS;
-    int acc_steps = 0, att_steps = 0;
-    for (int call = 0; call < a.steps && status == SUCCESS; ++call) {
-      const double dt_total = a.dt;
-      const double h_min = dt_total * 1.0e-15;
-      double t = 0.0, dt = dt_total * a.seed;
-      bool reject_last = false, reject_more = false;
-      long attempts = 0;
-      while (t < dt_total) {
-        if (a.max_attempts > 0 && attempts >= a.max_attempts) {
-          status = ERR_CONVERGENCE_FAILURE;
-          break;
-        }
-        ++attempts;
-        ++att_steps;
-        dt = fmin(dt, dt_total - t);
-        const double inv_g_dt = 1.0 / (Tab::G * dt);
-        // An opaque copy of the lane id: the generated code tests hundreds of
-        // constant lane masks, and with a visible loop-invariant lane id the
-        // compiler hoists every resulting exec mask out of the step loop and
-        // spills them.  Recomputing a mask next to its use is cheaper.
-        int ln = lane;
-        asm volatile("" : "+v"(ln));
-        double Sx[NB], K[S][NB], F[NB], rdiag[NB];
-#pragma unroll
-        for (int b = 0; b < NB; ++b) Sx[b] = x[b];
-        store_y(Sx);
-        assemble_w(inv_g_dt);
-        C_::factor(lu, rb, ln);
-#pragma unroll
-        for (int b = 0; b < NB; ++b) rdiag[b] = valid[b] ? 1.0 / lu[rb[b] + b * 64 + lane] : 0.0;
-        // Stage loop kept rolled so that one copy of the F and solve code serves
-        // every stage.  Coefficients are read by the (uniform) stage index; a
-        // zero coefficient skips its term and a unit one adds K bare, exactly
-        // the textual form of the stage sums.
-#pragma unroll 1
-        for (int s = 0; s < S; ++s) {
-          double wv[NB];
-          const int base = s * (s - 1) / 2;
-          if (s == 0 || Tab::NEWF[s]) {
-            if (s > 0) {
-              double Y[NB];
-#pragma unroll
-              for (int b = 0; b < NB; ++b) {
-                double v = Sx[b];
-#pragma unroll
-                for (int j = 0; j < S - 1; ++j) {
-                  const double c = j < s ? Tab::A[base + j] : 0.0;
-                  if (c == 0.0) continue;
-                  v = (c == 1.0) ? v + K[j][b] : v + c * K[j][b];
-                }
-                Y[b] = v;
-              }
-              store_y(Y);
-            }
-            eval_rates(F);
-          }
-#pragma unroll
-          for (int b = 0; b < NB; ++b) {
-            double v = F[b];
-#pragma unroll
-            for (int j = 0; j < S - 1; ++j) {
-              const double c = j < s ? Tab::C[base + j] : 0.0;
-              if (c == 0.0) continue;
-              v = v + (c / dt) * K[j][b];
-            }
-            wv[b] = valid[b] ? v : 0.0;
-          }
-          int ls = ln;
-          asm volatile("" : "+v"(ls));
-          double xs[NB];
-          C_::forward(lu, rb, wv, ls);
-          C_::backward(lu, rb, wv, rdiag, xs, ls);
-#pragma unroll
-          for (int j = 0; j < S; ++j)
-            if (j == s) {
-#pragma unroll
-              for (int b = 0; b < NB; ++b) K[j][b] = xs[b];
-            }
-        }
-        // error norm: rows summed in logical order (fixed species add zeros)
-        double term[NB];
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          double ynew = Sx[b], yerr = 0.0;
-          bool first = true;
-#pragma unroll
-          for (int j = 0; j < S; ++j) {
-            const double m = Tab::M[j];
-            if (m == 0.0) continue;
-            ynew = ynew + ((m == 1.0) ? K[j][b] : m * K[j][b]);
-          }
-#pragma unroll
-          for (int j = 0; j < S; ++j) {
-            const double e = Tab::E[j];
-            if (e == 0.0) continue;
-            const double tt = (e == 1.0) ? K[j][b] : e * K[j][b];
-            yerr = first ? tt : yerr + tt;
-            first = false;
-          }
-          const double sci = atol[b] + rtol[b] * fmax(fabs(x[b]), fabs(ynew));
-          term[b] = valid[b] ? (yerr / sci) * (yerr / sci) : 0.0;
-        }
-        double err_sq = 0.0;
-        for (int i = 0; i < NV; ++i) err_sq += readlane_d(bank<NB>(term, i), i & 63);
-        double err = sqrt(err_sq / W::N);
-        err = fmax(err, 1.0e-10);
-        const double factor_ = step_factor<Tab>(err);
-        if (err <= 1.0) {
-#pragma unroll
-          for (int b = 0; b < NB; ++b) {
-            double msum = 0.0;
-            bool first = true;
-#pragma unroll
-            for (int j = 0; j < S; ++j) {
-              const double m = Tab::M[j];
-              if (m == 0.0) continue;
-              const double tt = (m == 1.0) ? K[j][b] : m * K[j][b];
-              msum = first ? tt : msum + tt;
-              first = false;
-            }
-            x[b] += msum;
-          }
-          t += dt;
-          ++acc_steps;
-          double next_dt = fmax(h_min, fmin(dt * factor_, dt_total));
-          if (reject_last) next_dt = fmin(next_dt, dt);
-          reject_last = false;
-          reject_more = false;
-          dt = next_dt;
-        } else {
-          dt *= reject_more ? 0.1 : factor_;
-          reject_more = reject_last;
-          reject_last = true;
-        }
-      }
-    }
-    // ---------------- store ----------------
-    bool fin = true;
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      if (valid[b]) a.conc[cell + a.stride * (long)T_::VARPHYS[b * 64 + lane]] = x[b];
-      fin = fin && (!valid[b] || isfinite(x[b]));
-    }
-    const bool all_fin = __ballot(!fin) == 0ull;
-    if (lane == 0) {
-      if (status == SUCCESS && !all_fin) status = ERR_NUMERICAL_FAILURE;
-      if (a.status) a.status[cell] = status;
-      if (a.accepted) a.accepted[cell] += acc_steps;
-      if (a.attempted) a.attempted[cell] += att_steps;
-    }
-  }
+  for (long cell = first; cell < a.ncells; cell += gridDim.x) wave_integrate_cell<W, Tab>(a, cell, lds, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -511,7 +514,9 @@ int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
 
 }  // namespace mkpp
 
-#define MKPP_WAVE_MECHANISM(NAME, F0, F1, F2, F3, F4)                                                         \
+// The table bundle of a generated wave mechanism (any unit that runs its kernel
+// needs it: the mechanism's own unit and the batch unit).
+#define MKPP_WAVE_TABLES(NAME)                                                                              \
   namespace mkpp {                                                                                          \
   template <>                                                                                               \
   struct WaveTables<gen::NAME##_wave::Wave> {                                                               \
@@ -556,6 +561,12 @@ int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
     static constexpr const uint16_t* CW_ADDR = gen::NAME##_wave::CW_ADDR;                                   \
     using Code = gen::NAME##_wave::Code;                                                                    \
   };                                                                                                        \
+  }
+
+// Registers a generated wave mechanism: defines mkpp_ops_<name>().
+#define MKPP_WAVE_MECHANISM(NAME, F0, F1, F2, F3, F4)                                                         \
+  MKPP_WAVE_TABLES(NAME)                                                                                    \
+  namespace mkpp {                                                                                          \
   static int wave_eval_rates_##NAME(const EvalArgs& a, hipStream_t s) {                                     \
     return launch_wave_eval<gen::NAME##_wave::Wave>(a, s, 0);                                               \
   }                                                                                                         \

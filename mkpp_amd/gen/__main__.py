@@ -4,6 +4,7 @@ Writes, per bundled mechanism:
   mkpp_amd/plans/<mech>.json      plan (groups, slots, LU schedule, flop counts)
   oracle/tables/<mech>.tab        the same plan for the C oracle
   mkpp_amd/csrc/gen/<mech>.hip    HIP/CDNA4 kernels
+and the persistent batch units over all of them (mkpp_amd/csrc/gen/batch_*).
 """
 
 import argparse
@@ -25,6 +26,7 @@ def main(argv=None):
     os.makedirs(os.path.join(root, "mkpp_amd", "plans"), exist_ok=True)
     os.makedirs(os.path.join(root, "oracle", "tables"), exist_ok=True)
     compiled = []
+    engines = {}
     for name in args.mechanisms:
         mech = load_mechanism(name)
         p = planmod.build(mech)
@@ -37,14 +39,22 @@ def main(argv=None):
             nv = len(p["species"]) - sum(p["fixed"])
             if nv <= emit_hip.LANE_MAX_VARIABLE_SPECIES:
                 emit_hip.write(mech, p, outdir)
+                engines[name] = (name, "lane", 0)
             else:
                 emit_wave.write(mech, p, outdir)
+                t = emit_wave.build_tables(mech, p)
+                lds = (t["span"] + 1) + t["ny"] + (max(t["ng"], t["npart"]) + 1) + (t["nv"] + 1)
+                engines[name] = (name, "wave", lds)
         if not args.no_hip and name in BUNDLED:
             compiled.append(name)
         print(f"{name}: N={len(p['species'])} groups={len(p['groups'])} J={len(p['j_entries'])} "
               f"LU={len(p['lu_locations'])} step_flops(rodas3)={p['step_flops']['rodas3']}", file=sys.stderr)
     if compiled:
-        registry([m for m in BUNDLED if m in compiled])
+        names = [m for m in BUNDLED if m in compiled]
+        registry(names)
+        from . import emit_batch
+
+        emit_batch.write([engines[m] for m in names], os.path.join(root, "mkpp_amd", "csrc", "gen"))
 
 
 def registry(names):

@@ -67,6 +67,23 @@ _SIGS = {
                                   _vp], C.c_int),
 }
 
+
+
+class BatchSegment(C.Structure):
+    """mkpp_amd_batch_segment (include/mkpp_amd.h)."""
+
+    _fields_ = [("mechanism", C.c_char_p), ("conc", _vp), ("num_cells", C.c_long), ("stride", C.c_long),
+                ("temp", _vp), ("air_density", _vp), ("rate_parameters", _vp), ("parameter_stride", C.c_long),
+                ("atol", _vp), ("rtol", _vp), ("status", _vp), ("accepted", _vp), ("attempted", _vp)]
+
+
+BATCH_MAX_SEGMENTS = 8
+BATCH_WORKSPACE_BYTES = 32
+
+_SIGS["mkpp_amd_device_integrate_batch"] = ([C.POINTER(BatchSegment), C.c_int, C.c_double, C.c_int, C.c_int,
+                                             C.c_double, C.c_long, C.c_int, C.c_int, _vp, _vp], C.c_int)
+_SIGS["mkpp_amd_batch_capacity"] = ([C.c_int, _ip, _ip], C.c_int)
+
 _lib = None
 
 
@@ -249,3 +266,53 @@ def device_evaluate(mechanism: str, what: str, conc, temp=None, air_density=None
     if rc != MKPP_SUCCESS:
         raise MkppError(rc, "device_evaluate")
     return out if what == "rates" else out.permute(1, 0, 2)
+
+
+def batch_capacity(method: str = "rodas3"):
+    """(workgroups one CU holds, CU count) for the persistent batch kernel."""
+    per_cu, cus = C.c_int(), C.c_int()
+    rc = lib().mkpp_amd_batch_capacity(METHODS[method], C.byref(per_cu), C.byref(cus))
+    if rc != MKPP_SUCCESS:
+        raise MkppError(rc, "batch_capacity")
+    return per_cu.value, cus.value
+
+
+def device_integrate_batch(segments, dt: float, steps: int = 1, method: str = "rodas3", seed: float = 1.0e-6,
+                           max_attempts: int = 100000, cells_per_lane: int = 1, workgroups: int = 0,
+                           workspace=None, stream=None):
+    """One persistent launch over several mechanisms' device-resident cells.
+
+    segments: list of dicts with keys mechanism, conc (species, cells) and the
+    optional temp, air_density, rate_parameters, atol, rtol (required),
+    status, accepted, attempted -- torch CUDA tensors as for device_integrate.
+    workspace: int32 CUDA tensor of >= 8 elements (allocated here if None).
+    Enqueued on `stream`; returns immediately.
+    """
+    import torch
+
+    keep = []
+    arr = (BatchSegment * len(segments))()
+    for i, g in enumerate(segments):
+        conc = g["conc"]
+        assert conc.is_cuda and conc.dtype == torch.float64 and conc.is_contiguous()
+        n, ncell = conc.shape
+        name = g["mechanism"].encode()
+        keep.append(name)
+        rp = g.get("rate_parameters")
+        arr[i] = BatchSegment(name, conc.data_ptr(), ncell, ncell, _addr(g.get("temp")), _addr(g.get("air_density")),
+                              _addr(rp), ncell if rp is not None else 0, _addr(g.get("atol")), _addr(g.get("rtol")),
+                              _addr(g.get("status")), _addr(g.get("accepted")), _addr(g.get("attempted")))
+    if workspace is None:
+        workspace = torch.zeros(BATCH_WORKSPACE_BYTES // 4, dtype=torch.int32, device=segments[0]["conc"].device)
+    assert workspace.is_cuda and workspace.numel() * workspace.element_size() >= BATCH_WORKSPACE_BYTES
+    s = stream if stream is not None else torch.cuda.current_stream()
+    rc = lib().mkpp_amd_device_integrate_batch(arr, len(segments), float(dt), int(steps), METHODS[method],
+                                               float(seed), int(max_attempts), int(cells_per_lane), int(workgroups),
+                                               C.c_void_p(workspace.data_ptr()), C.c_void_p(s.cuda_stream))
+    if rc != MKPP_SUCCESS:
+        raise MkppError(rc, "device_integrate_batch")
+    return workspace
+
+
+def _addr(t):
+    return None if t is None else t.data_ptr()

@@ -13,6 +13,9 @@ tolerances the configuration integrates with.
                     NOx / VOCs rescaled log-uniformly, photolysis scaled by cos(SZA) and
                     half of the cells at night
 * carbon_mixed    - carbon, Rodas3 adaptive, T-dependent rate constants
+* batch_mixed     - the batching workload: saprc99 (NOx, VOCs 1e8-1e13, half night),
+                    carbon and small_strato (half night) cells, Rodas3 adaptive
+                    at rtol 1e-3 for all three in one launch
 """
 
 from __future__ import annotations
@@ -143,3 +146,26 @@ def carbon_mixed(num_cells: int = 4096, seed: int = 4) -> dict:
     return {"mechanism": "carbon", "conc": conc, "rate_parameters": np.zeros((0, num_cells)), "temp": T,
             "pressure": p, "air_density": air, "method": "rodas3", "dt": 3600.0, "steps": 1, "seed": 1.0e-6,
             "atol": 1.0e3, "rtol": 1.0e-3}
+
+
+def batch_mixed(num_saprc99: int = 131072, num_carbon: int = 65536, num_small_strato: int = 65536,
+                seed: int = 4) -> list:
+    """Three workloads of deliberately uneven stiffness for one persistent launch.
+
+    saprc99 urban/rural cells with NOx and VOCs log-uniform over [1e8, 1e13]
+    and half of them at night; carbon cells (no photolysis); small_strato cells
+    with half of them at night.  All integrate 3600 s with Rodas3 at rtol 1e-3,
+    atol 1e3.  Accepted steps per cell then range from 9 (the floor set by the
+    controller: growth at most x6 per step from the 1e-6 * 3600 s seed) to
+    about 70 (sunlit saprc99 cells)."""
+    sap = saprc99_urban(num_saprc99, seed=seed, nox_range=(1e8, 1e13), voc_range=(1e8, 1e13))
+    car = carbon_mixed(num_carbon, seed=seed + 1)
+    sst = small_strato(num_small_strato, seed=seed + 2)
+    rng = np.random.default_rng(seed + 3)
+    night = rng.uniform(0.0, 1.0, num_small_strato) < 0.5
+    sst["rate_parameters"][:, night] = 0.0
+    sst["night"] = night
+    out = []
+    for w in (sap, car, sst):
+        out.append(dict(w, method="rodas3", dt=3600.0, steps=1, seed=1.0e-6, atol=1.0e3, rtol=1.0e-3))
+    return out
