@@ -145,6 +145,43 @@ int mkpp_amd_device_integrate_batch(const mkpp_amd_batch_segment* segments, int 
  * the CU count of the current device (the default grid is their product). */
 int mkpp_amd_batch_capacity(int method, int* per_cu, int* num_cus);
 
+/* ---- 5. differentiation (wave-per-cell mechanisms, Rodas3 and Ros4) ----
+ * Replaces the reference's frozen-Jacobian checkpoint / adjoint templates
+ * (solver_variants/integrate_fwd_checkpoint.j2:37-295, integrate_adj.j2:50-156)
+ * with the exact adjoint of the discrete Rosenbrock map over the accepted step
+ * sequence of a forward run.  Rate constants are per flux group (reactions
+ * with the same rate law and reactants share one); k arrays are
+ * [num_groups x k_stride] per cell, k_stride >= num_cells.  Checkpoints are
+ * max_steps * num_variable doubles per cell (ckpt) and max_steps doubles
+ * (hs); nsteps[cell] receives the accepted step count.  Device pointers;
+ * enqueued on `stream`; allocates nothing. */
+#define MKPP_CELL_ERR_CHECKPOINT_OVERFLOW 5
+
+int mkpp_amd_diff_info(const char* mechanism, int* num_groups, int* num_variable);
+/* k[cell + k_stride*group] from the rate laws at the cells' T, air density and
+ * rate parameters. */
+int mkpp_amd_device_rate_constants(const char* mechanism, const double* conc, long num_cells, long stride,
+                                   const double* temp, const double* air_density, const double* rate_parameters,
+                                   long parameter_stride, double* k, long k_stride, void* stream);
+/* replay = 0: adaptive forward integration over dt (one interval) that records
+ * every accepted step; replay = 1: integrates the recorded step sizes again
+ * without error control (the frozen-step map).  k may be NULL (rate laws). */
+int mkpp_amd_device_integrate_checkpointed(const char* mechanism, double* conc, long num_cells, long stride,
+                                           const double* temp, const double* air_density,
+                                           const double* rate_parameters, long parameter_stride, const double* k,
+                                           long k_stride, double dt, int method, double initial_step_seed,
+                                           const double* atol, const double* rtol, int* status, int* accepted,
+                                           int* attempted, long max_attempts, double* ckpt, double* hs, int* nsteps,
+                                           int max_steps, int replay, void* stream);
+/* lambda[cell + stride*species]: in dL/dy at the end of the interval, out
+ * dL/dy at its start (fixed species included); kbar[cell + k_stride*group]
+ * out dL/dk.  conc supplies the fixed species (any time: they do not change). */
+int mkpp_amd_device_adjoint(const char* mechanism, const double* conc, long num_cells, long stride,
+                            const double* temp, const double* air_density, const double* rate_parameters,
+                            long parameter_stride, const double* k, long k_stride, int method, const double* ckpt,
+                            const double* hs, const int* nsteps, int max_steps, double* lambda, double* kbar,
+                            void* stream);
+
 #ifdef __cplusplus
 }
 #endif

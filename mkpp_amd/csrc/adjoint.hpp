@@ -1,0 +1,521 @@
+// Exact discrete adjoint of the wave-per-cell Rosenbrock integrator.
+//
+// The reference's differentiation templates (solver_variants/
+// integrate_fwd_checkpoint.j2:37-295, integrate_adj.j2:50-156) freeze the
+// Jacobian and are never run numerically; this is the adjoint of the exact
+// discrete map instead.  The forward pass (wave_integrate_cell, MODE 1) keeps
+// the state y_n and step size h_n of every accepted step.  Going backwards
+// over those steps, one wavefront per cell
+//   1. recomputes W = I/(gamma h) - J(y_n), its LU and the stage vectors K_s
+//      from the checkpoint (the forward code path, error control dropped);
+//   2. runs the stage recursion backwards:
+//        Kbar_s = M_s lambda (+ contributions of later stages)
+//        rbar_s = W^-T Kbar_s                       (transposed LU solves)
+//        Fbar of the stage that evaluated F += rbar_s, Kbar_j += C_sj/h rbar_s
+//        at a stage that evaluated F(Y_s): Ybar = J(Y_s)^T Fbar, kbar += dF/dk^T Fbar,
+//        ybar += Ybar, Kbar_j += A_sj Ybar
+//   3. adds the Jacobian's own dependence on y_n and k:
+//        ybar += sum_s d/dy [rbar_s^T J(y_n) K_s]  (second derivatives of the fluxes)
+//        kbar += sum_s d/dk [rbar_s^T J(y_n) K_s]
+//      and lambda_n = ybar.
+// Step sizes are frozen (the accepted sequence of the forward run), so this
+// is the gradient of the map that a replay of those steps computes.
+//
+// Fluxes are mass-action monomials r_g = k_g prod_e y_e^q_e (at most three
+// entries), so J^T u and the second-derivative terms are sums over (group,
+// entry) partial slots, scattered to species through the generated SP lists.
+// Fixed species take part in the fluxes but not in the state: their adjoint
+// accumulates over all steps.  Rate constants are per flux group.
+#pragma once
+#include "wave_engine.hpp"
+
+namespace mkpp {
+
+// LDS doubles of the adjoint: the forward layout plus rbar and K of every
+// stage, the u vector of a J^T pass and the fixed-species adjoint.
+template <class W, class Tab>
+constexpr int adjoint_lds_doubles() {
+  return W::LDS_DOUBLES + 2 * Tab::S * (W::NV + 1) + Tab::S * W::NY + (W::NV + 1) + (W::NF + 1);
+}
+
+template <class W, class Tab>
+__device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const AdjArgs& g, long cell, double* lds,
+                                                  int lane) {
+  using T_ = WaveTables<W>;
+  using C_ = typename T_::Code;
+  constexpr int NV = W::NV, NB = W::NB, NG = W::NG, NGP = (NG + 63) / 64, S = Tab::S, NY = W::NY, NF = W::NF;
+  double* lu = lds;
+  double* ylds = lu + (W::SPAN + 1);
+  double* scr = ylds + NY;
+  double* facc = scr + W::NSCR;
+  double* rbv = facc + (NV + 1);     // [S][NV+1] rbar_s by row
+  double* kv = rbv + S * (NV + 1);   // [S][NY]   K_s by y slot (fixed species and ONE: 0)
+  double* kbv = kv + S * NY;         // [S][NV+1] Kbar_s by row
+  double* uv = kbv + S * (NV + 1);   // [NV+1]    u of a J^T pass by row
+  double* fbar = uv + (NV + 1);      // [NF]      adjoint of the fixed species
+
+  int rb[NB];
+  bool valid[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int row = b * 64 + lane;
+    valid[b] = row < NV;
+    rb[b] = valid[b] ? (int)T_::R[row] : 0;
+  }
+  const int nst = g.nsteps[cell];
+
+  // ---- environment, fixed species, rate constants (as the forward pass) ----
+  if (lane < NF) ylds[NV + lane] = a.conc[cell + a.stride * (long)T_::FIXPHYS[lane]];
+  if (lane == 0) ylds[W::ONE] = 1.0;
+  const double Tk = a.temp ? a.temp[cell] : 300.0;
+  const double rho = a.air_density ? a.air_density[cell] : 2.4476e19;
+  const double Mtb = (W::THIRD_BODY >= 0) ? a.conc[cell + a.stride * (long)(W::THIRD_BODY >= 0 ? W::THIRD_BODY : 0)] : rho;
+  double kreg[NGP], kbar[NGP];
+#pragma unroll
+  for (int p = 0; p < NGP; ++p) {
+    const int gi = p * 64 + lane;
+    kreg[p] = 0.0;
+    kbar[p] = 0.0;
+    if (gi < NG) {
+      if (g.k != nullptr) {
+        kreg[p] = g.k[cell + g.kstride * (long)gi];
+      } else {
+        const int law = T_::LAW[gi];
+        const int ph = T_::PHOTO[gi];
+        const double jv = (law == 0 && a.jvals != nullptr) ? a.jvals[cell + a.jstride * ph] : 0.0;
+        kreg[p] = wave_rate(law, &T_::PARAMS[gi * 7], Tk, Mtb, jv);
+      }
+    }
+  }
+
+  // ---- lambda = dL/dy(T) ----
+  double lam[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int row = b * 64 + lane;
+    lam[b] = valid[b] ? g.lambda[cell + a.stride * (long)T_::VARPHYS[row]] : 0.0;
+  }
+  if (lane < NF) fbar[lane] = g.lambda[cell + a.stride * (long)T_::FIXPHYS[lane]];
+  for (int i = lane; i < S * NY; i += 64) kv[i] = 0.0;
+
+  auto store_y = [&](const double (&y)[NB]) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      if (valid[b]) ylds[b * 64 + lane] = y[b];
+  };
+
+  auto eval_rates = [&](double (&f)[NB]) {
+#pragma unroll
+    for (int p = 0; p < NGP; ++p) {
+      const int gi = p * 64 + lane;
+      if (gi < NG) {
+        const uint8_t* fa = &T_::GFAC[gi * 3];
+        scr[gi] = kreg[p] * (((1.0 * ylds[fa[0]]) * ylds[fa[1]]) * ylds[fa[2]]);
+      }
+    }
+    if (lane == 0) scr[NG] = 0.0;
+    for (int r = lane; r <= NV; r += 64) facc[r] = 0.0;
+#pragma unroll
+    for (int p = 0; p < C_::NFP; ++p) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < C_::TF; ++q) {
+        const uint32_t tm = T_::CF_TERM[(p * C_::TF + q) * 64 + lane];
+        acc = fma(T_::COEF[tm >> 16], scr[tm & 0xFFFFu], acc);
+      }
+      const int row = T_::CF_ROW[p * 64 + lane];
+      if (row != 255) facc[row] = facc[row] + acc;
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) f[b] = valid[b] ? facc[b * 64 + lane] : 0.0;
+  };
+
+  // reactant powers of group gi at the state in ylds: pw = y^q, pm1 = q y^(q-1),
+  // pm2 = q (q-1) y^(q-2) (zero where the power is too small)
+  auto monomial = [&](int gi, double (&pw)[3], double (&pm1)[3], double (&pm2)[3]) {
+    const uint8_t* s = &T_::GPOWS[gi * 3];
+    const uint8_t* q = &T_::GPOWP[gi * 3];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      const double y = ylds[s[e]];
+      const double y2 = y * y;
+      const int qe = q[e];
+      pw[e] = qe == 0 ? 1.0 : (qe == 1 ? y : (qe == 2 ? y2 : y2 * y));
+      pm1[e] = qe == 0 ? 0.0 : (qe == 1 ? 1.0 : (qe == 2 ? 2.0 * y : 3.0 * y2));
+      pm2[e] = qe <= 1 ? 0.0 : (qe == 2 ? 2.0 : 6.0 * y);
+    }
+  };
+
+  auto assemble_w = [&](double inv_g_dt) {
+#pragma unroll
+    for (int p = 0; p < NGP; ++p) {
+      const int gi = p * 64 + lane;
+      if (gi < NG) {
+        const int np = T_::GNPOW[gi];
+        const uint8_t* s = &T_::GPOWS[gi * 3];
+        const uint8_t* q = &T_::GPOWP[gi * 3];
+        const int off = T_::POFF[gi];
+        double yv[3], pw[3], pm1[3];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          yv[e] = ylds[s[e]];
+          const double y2 = yv[e] * yv[e];
+          pw[e] = q[e] == 0 ? 1.0 : (q[e] == 1 ? yv[e] : (q[e] == 2 ? y2 : y2 * yv[e]));
+          pm1[e] = q[e] <= 1 ? 1.0 : (q[e] == 2 ? yv[e] : y2);
+        }
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          double t = (kreg[p] * (double)q[e]) * pm1[e];
+#pragma unroll
+          for (int o = 0; o < 3; ++o)
+            if (o != e) t = t * pw[o];
+          if (e < np) scr[off + e] = t;
+        }
+      }
+    }
+    if (lane == 0) scr[W::NPART] = 0.0;
+    for (int i = lane; i <= W::SPAN; i += 64) lu[i] = 0.0;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      if (valid[b]) lu[rb[b] + b * 64 + lane] = inv_g_dt;
+#pragma unroll
+    for (int p = 0; p < C_::NWP; ++p) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < C_::TW; ++q) {
+        const uint32_t tm = T_::CW_TERM[(p * C_::TW + q) * 64 + lane];
+        acc = fma(T_::COEF[tm >> 16], scr[tm & 0xFFFFu], acc);
+      }
+      const int ad = T_::CW_ADDR[p * 64 + lane];
+      if (ad != 0xFFFF) lu[ad] = lu[ad] - acc;
+    }
+  };
+
+  // nu_g . u for every group of this lane, u by row in LDS
+  auto nu_dot = [&](int gi, const double* u) {
+    double w = 0.0;
+    for (int t = T_::GNVOFS[gi]; t < T_::GNVOFS[gi + 1]; ++t) w = fma(T_::COEF[T_::GNVC[t]], u[T_::GNVR[t]], w);
+    return w;
+  };
+
+  // scr[partial slot] summed per species slot: rows into out[], fixed species into fbar
+  auto scatter = [&](double (&out)[NB]) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int l = b * 64 + lane;
+      double acc = 0.0;
+      if (l < NV + NF)
+        for (int t = T_::SPOFS[l]; t < T_::SPOFS[l + 1]; ++t) acc += scr[T_::SPI[t]];
+      out[b] = valid[b] ? acc : 0.0;
+      if (l >= NV && l < NV + NF) fbar[l - NV] += acc;
+    }
+  };
+
+  // J(Y)^T u at the state in ylds (u by row in uv), plus kbar += u . dF/dk
+  auto jt_pass = [&](double (&out)[NB]) {
+#pragma unroll
+    for (int p = 0; p < NGP; ++p) {
+      const int gi = p * 64 + lane;
+      if (gi < NG) {
+        const double w = nu_dot(gi, uv);
+        double pw[3], pm1[3], pm2[3];
+        monomial(gi, pw, pm1, pm2);
+        kbar[p] = fma(w, (pw[0] * pw[1]) * pw[2], kbar[p]);
+        const int np = T_::GNPOW[gi];
+        const int off = T_::POFF[gi];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          double d = pm1[e];
+#pragma unroll
+          for (int o = 0; o < 3; ++o)
+            if (o != e) d *= pw[o];
+          if (e < np) scr[off + e] = (w * kreg[p]) * d;
+        }
+      }
+    }
+    scatter(out);
+  };
+
+  // second-derivative terms at y_n: sum_s d/dy [rbar_s^T J K_s] and d/dk
+  auto hess_pass = [&](double (&out)[NB]) {
+#pragma unroll
+    for (int p = 0; p < NGP; ++p) {
+      const int gi = p * 64 + lane;
+      if (gi < NG) {
+        double pw[3], pm1[3], pm2[3];
+        monomial(gi, pw, pm1, pm2);
+        const uint8_t* sl = &T_::GPOWS[gi * 3];
+        // first and second derivatives of the monomial (without k)
+        double d1[3], d2[3][3];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          double d = pm1[e];
+#pragma unroll
+          for (int o = 0; o < 3; ++o)
+            if (o != e) d *= pw[o];
+          d1[e] = d;
+#pragma unroll
+          for (int f = 0; f < 3; ++f) {
+            double h;
+            if (f == e) {
+              h = pm2[e];
+#pragma unroll
+              for (int o = 0; o < 3; ++o)
+                if (o != e) h *= pw[o];
+            } else {
+              h = pm1[e] * pm1[f];
+#pragma unroll
+              for (int o = 0; o < 3; ++o)
+                if (o != e && o != f) h *= pw[o];
+            }
+            d2[e][f] = h;
+          }
+        }
+        double val[3] = {0.0, 0.0, 0.0};
+        double kb = 0.0;
+        for (int s = 0; s < S; ++s) {
+          const double w = nu_dot(gi, rbv + s * (NV + 1));
+          double kvs[3];
+#pragma unroll
+          for (int e = 0; e < 3; ++e) kvs[e] = kv[s * NY + sl[e]];
+          double dk = 0.0;
+#pragma unroll
+          for (int e = 0; e < 3; ++e) {
+            dk = fma(kvs[e], d1[e], dk);
+            double de = 0.0;
+#pragma unroll
+            for (int f = 0; f < 3; ++f) de = fma(kvs[f], d2[f][e], de);
+            val[e] = fma(w, de, val[e]);
+          }
+          kb = fma(w, dk, kb);
+        }
+        kbar[p] += kb;
+        const int np = T_::GNPOW[gi];
+        const int off = T_::POFF[gi];
+#pragma unroll
+        for (int e = 0; e < 3; ++e)
+          if (e < np) scr[off + e] = kreg[p] * val[e];
+      }
+    }
+    scatter(out);
+  };
+
+  for (int n = nst - 1; n >= 0; --n) {
+    const double dt = g.hs[cell * (long)g.maxsteps + n];
+    const double* ck = g.ckpt + (cell * (long)g.maxsteps + n) * NV;
+    double x[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) x[b] = valid[b] ? ck[b * 64 + lane] : 0.0;
+    const double inv_g_dt = 1.0 / (Tab::G * dt);
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    double K[S][NB], F[NB], rdiag[NB];
+    store_y(x);
+    assemble_w(inv_g_dt);
+    C_::factor(lu, rb, ln);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) rdiag[b] = valid[b] ? 1.0 / lu[rb[b] + b * 64 + lane] : 0.0;
+
+    // ---- forward stages, recomputed from the checkpoint ----
+#pragma unroll 1
+    for (int s = 0; s < S; ++s) {
+      double wv[NB];
+      const int base = s * (s - 1) / 2;
+      if (s == 0 || Tab::NEWF[s]) {
+        if (s > 0) {
+          double Y[NB];
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            double v = x[b];
+#pragma unroll
+            for (int j = 0; j < S - 1; ++j) {
+              const double c = j < s ? Tab::A[base + j] : 0.0;
+              if (c == 0.0) continue;
+              v = (c == 1.0) ? v + K[j][b] : v + c * K[j][b];
+            }
+            Y[b] = v;
+          }
+          store_y(Y);
+        }
+        eval_rates(F);
+      }
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        double v = F[b];
+#pragma unroll
+        for (int j = 0; j < S - 1; ++j) {
+          const double c = j < s ? Tab::C[base + j] : 0.0;
+          if (c == 0.0) continue;
+          v = v + (c / dt) * K[j][b];
+        }
+        wv[b] = valid[b] ? v : 0.0;
+      }
+      int ls = ln;
+      asm volatile("" : "+v"(ls));
+      double xs[NB];
+      C_::forward(lu, rb, wv, ls);
+      C_::backward(lu, rb, wv, rdiag, xs, ls);
+#pragma unroll
+      for (int j = 0; j < S; ++j)
+        if (j == s) {
+#pragma unroll
+          for (int b = 0; b < NB; ++b) K[j][b] = xs[b];
+        }
+    }
+    // K_s by y slot for the second-derivative pass
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        if (valid[b]) kv[s * NY + b * 64 + lane] = K[s][b];
+
+    // ---- backward stage recursion (K and Kbar live in LDS, by row) ----
+    double yb[NB], pend[NB];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        if (valid[b]) kbv[s * (NV + 1) + b * 64 + lane] = Tab::M[s] * lam[b];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      yb[b] = lam[b];
+      pend[b] = 0.0;
+    }
+#pragma unroll 1
+    for (int s = S - 1; s >= 0; --s) {
+      const int base = s * (s - 1) / 2;
+      double kbs[NB], z[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) kbs[b] = valid[b] ? kbv[s * (NV + 1) + b * 64 + lane] : 0.0;
+      int ls = ln;
+      asm volatile("" : "+v"(ls));
+      C_::tforward(lu, kbs, rdiag, z, ls);
+      C_::tbackward(lu, z, ls);
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        if (!valid[b]) z[b] = 0.0;
+        if (valid[b]) rbv[s * (NV + 1) + b * 64 + lane] = z[b];
+        pend[b] += z[b];
+      }
+      for (int j = 0; j < s; ++j) {
+        const double c = Tab::C[base + j];
+        if (c == 0.0) continue;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          if (valid[b]) {
+            double* kb = &kbv[j * (NV + 1) + b * 64 + lane];
+            *kb = fma(c / dt, z[b], *kb);
+          }
+      }
+      if (s == 0 || Tab::NEWF[s]) {
+        double Y[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) Y[b] = x[b];
+        for (int j = 0; j < s; ++j) {
+          const double c = Tab::A[base + j];
+          if (c == 0.0) continue;
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            const double kj = valid[b] ? kv[j * NY + b * 64 + lane] : 0.0;
+            Y[b] = (c == 1.0) ? Y[b] + kj : Y[b] + c * kj;
+          }
+        }
+        store_y(Y);
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          if (valid[b]) uv[b * 64 + lane] = pend[b];
+        double Yb[NB];
+        jt_pass(Yb);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          yb[b] += Yb[b];
+          pend[b] = 0.0;
+        }
+        for (int j = 0; j < s; ++j) {
+          const double c = Tab::A[base + j];
+          if (c == 0.0) continue;
+#pragma unroll
+          for (int b = 0; b < NB; ++b)
+            if (valid[b]) {
+              double* kb = &kbv[j * (NV + 1) + b * 64 + lane];
+              *kb = fma(c, Yb[b], *kb);
+            }
+        }
+      }
+    }
+    // ---- the Jacobian's dependence on y_n (ylds holds y_n after stage 0) ----
+    double Hb[NB];
+    hess_pass(Hb);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) lam[b] = yb[b] + Hb[b];
+  }
+
+  // ---- outputs ----
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+    if (valid[b]) g.lambda[cell + a.stride * (long)T_::VARPHYS[b * 64 + lane]] = lam[b];
+  if (lane < NF) g.lambda[cell + a.stride * (long)T_::FIXPHYS[lane]] = fbar[lane];
+#pragma unroll
+  for (int p = 0; p < NGP; ++p)
+    if (p * 64 + lane < NG) g.kbar[cell + g.kstride * (long)(p * 64 + lane)] = kbar[p];
+}
+
+template <class W, class Tab>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) wave_adjoint_kernel(DeviceArgs a, AdjArgs g) {
+  extern __shared__ double lds[];
+  const long first = (long)(blockIdx.x % 8u) * (gridDim.x / 8u) + blockIdx.x / 8u;
+  for (long cell = first; cell < a.ncells; cell += gridDim.x) wave_adjoint_cell<W, Tab>(a, g, cell, lds, threadIdx.x);
+}
+
+template <class W, class Tab, int MODE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) wave_record_kernel(DeviceArgs a, AdjArgs g) {
+  extern __shared__ double lds[];
+  const long first = (long)(blockIdx.x % 8u) * (gridDim.x / 8u) + blockIdx.x / 8u;
+  for (long cell = first; cell < a.ncells; cell += gridDim.x)
+    wave_integrate_cell<W, Tab, MODE>(a, cell, lds, threadIdx.x, &g);
+}
+
+// Rate constants per flux group and cell (the values the laws give; the
+// differentiable inputs of the adjoint).
+template <class W>
+__global__ void __launch_bounds__(64) wave_rate_constants_kernel(DeviceArgs a, double* k, long kstride) {
+  using T_ = WaveTables<W>;
+  const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= a.ncells) return;
+  const double Tk = a.temp ? a.temp[cell] : 300.0;
+  const double rho = a.air_density ? a.air_density[cell] : 2.4476e19;
+  const double Mtb = (W::THIRD_BODY >= 0) ? a.conc[cell + a.stride * (long)(W::THIRD_BODY >= 0 ? W::THIRD_BODY : 0)] : rho;
+  for (int gi = 0; gi < W::NG; ++gi) {
+    const int law = T_::LAW[gi];
+    const int ph = T_::PHOTO[gi];
+    const double jv = (law == 0 && a.jvals != nullptr) ? a.jvals[cell + a.jstride * ph] : 0.0;
+    k[cell + kstride * (long)gi] = wave_rate(law, &T_::PARAMS[gi * 7], Tk, Mtb, jv);
+  }
+}
+
+// mode 0: rate constants only; 1: checkpointed forward; 2: replay; 3: adjoint.
+// Only Rodas3 and Ros4 are instantiated for differentiation.
+template <class W>
+int launch_wave_diff(int mode, const DeviceArgs& a, const AdjArgs& g, hipStream_t s) {
+  if (a.ncells <= 0) return 0;
+  const long cells = a.ncells < (1L << 30) ? a.ncells : (1L << 30);
+  const unsigned grid = (unsigned)((cells + 7) / 8 * 8);
+  if (mode == 0) {
+    wave_rate_constants_kernel<W><<<(unsigned)((a.ncells + 63) / 64), 64, 0, s>>>(a, g.kbar, g.kstride);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+  }
+  const size_t fwd_lds = sizeof(double) * W::LDS_DOUBLES;
+#define MKPP_DIFF_CASE(TAB)                                                                        \
+  if (mode == 1) wave_record_kernel<W, TAB, 1><<<grid, 64, fwd_lds, s>>>(a, g);                    \
+  else if (mode == 2) wave_record_kernel<W, TAB, 2><<<grid, 64, fwd_lds, s>>>(a, g);               \
+  else wave_adjoint_kernel<W, TAB><<<grid, 64, sizeof(double) * adjoint_lds_doubles<W, TAB>(), s>>>(a, g);
+  switch (a.method) {
+    case RODAS3: MKPP_DIFF_CASE(TabRodas3) break;
+    case ROS4: MKPP_DIFF_CASE(TabRos4) break;
+    default: return -1;
+  }
+#undef MKPP_DIFF_CASE
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace mkpp

@@ -634,3 +634,119 @@ int mkpp_amd_device_integrate_batch(const mkpp_amd_batch_segment* segments, int 
   }
   return L->launch(b, lds, workgroups, (hipStream_t)stream) == 0 ? MKPP_SUCCESS : MKPP_ERROR_INVALID_STATE;
 }
+
+// ---------------------------------------------------------------------------
+// Differentiation (mkpp_amd.h section 5)
+// ---------------------------------------------------------------------------
+namespace {
+const MechOps* diff_mechanism(const char* name, int* rc) {
+  const MechOps* o = mkpp::find_mechanism(name);
+  if (!name) *rc = MKPP_ERROR_NULL_POINTER;
+  else if (!o) *rc = MKPP_ERROR_NOT_FOUND;
+  else if (!o->diff) *rc = MKPP_ERROR_INVALID_ARGUMENT;
+  else *rc = MKPP_SUCCESS;
+  return *rc == MKPP_SUCCESS ? o : nullptr;
+}
+
+DeviceArgs diff_args(const double* conc, long num_cells, long stride, const double* temp, const double* air_density,
+                     const double* rate_parameters, long parameter_stride) {
+  DeviceArgs a{};
+  a.conc = const_cast<double*>(conc);
+  a.ncells = num_cells;
+  a.stride = stride;
+  a.temp = temp;
+  a.air_density = air_density;
+  a.jvals = rate_parameters;
+  a.jstride = parameter_stride;
+  a.steps = 1;
+  return a;
+}
+}  // namespace
+
+int mkpp_amd_diff_info(const char* mechanism, int* num_groups, int* num_variable) {
+  if (!num_groups || !num_variable) return MKPP_ERROR_NULL_POINTER;
+  int rc;
+  const MechOps* o = diff_mechanism(mechanism, &rc);
+  if (!o) return rc;
+  *num_groups = o->num_groups;
+  *num_variable = o->num_variable;
+  return MKPP_SUCCESS;
+}
+
+int mkpp_amd_device_rate_constants(const char* mechanism, const double* conc, long num_cells, long stride,
+                                   const double* temp, const double* air_density, const double* rate_parameters,
+                                   long parameter_stride, double* k, long k_stride, void* stream) {
+  int rc;
+  const MechOps* o = diff_mechanism(mechanism, &rc);
+  if (!o) return rc;
+  if (!conc || !k) return MKPP_ERROR_NULL_POINTER;
+  if (num_cells < 0 || stride < num_cells || k_stride < num_cells ||
+      (rate_parameters && parameter_stride < num_cells))
+    return MKPP_ERROR_INVALID_ARGUMENT;
+  DeviceArgs a = diff_args(conc, num_cells, stride, temp, air_density, rate_parameters, parameter_stride);
+  mkpp::AdjArgs g{};
+  g.kbar = k;  // mode 0 writes the rate constants through this pointer
+  g.kstride = k_stride;
+  return o->diff(0, a, g, (hipStream_t)stream) == 0 ? MKPP_SUCCESS : MKPP_ERROR_INVALID_STATE;
+}
+
+int mkpp_amd_device_integrate_checkpointed(const char* mechanism, double* conc, long num_cells, long stride,
+                                           const double* temp, const double* air_density,
+                                           const double* rate_parameters, long parameter_stride, const double* k,
+                                           long k_stride, double dt, int method, double initial_step_seed,
+                                           const double* atol, const double* rtol, int* status, int* accepted,
+                                           int* attempted, long max_attempts, double* ckpt, double* hs, int* nsteps,
+                                           int max_steps, int replay, void* stream) {
+  int rc;
+  const MechOps* o = diff_mechanism(mechanism, &rc);
+  if (!o) return rc;
+  if (!conc || !atol || !rtol || !ckpt || !hs || !nsteps) return MKPP_ERROR_NULL_POINTER;
+  if (num_cells < 0 || stride < num_cells || (rate_parameters && parameter_stride < num_cells) ||
+      (k && k_stride < num_cells) || !std::isfinite(dt) || dt <= 0.0 || !(initial_step_seed > 0.0) ||
+      max_steps < 1 || (method != mkpp::RODAS3 && method != mkpp::ROS4) || (replay != 0 && replay != 1))
+    return MKPP_ERROR_INVALID_ARGUMENT;
+  DeviceArgs a = diff_args(conc, num_cells, stride, temp, air_density, rate_parameters, parameter_stride);
+  a.dt = dt;
+  a.method = method;
+  a.seed = initial_step_seed;
+  a.atol = atol;
+  a.rtol = rtol;
+  a.status = status;
+  a.accepted = accepted;
+  a.attempted = attempted;
+  a.max_attempts = max_attempts;
+  mkpp::AdjArgs g{};
+  g.k = k;
+  g.kstride = k_stride;
+  g.ckpt = ckpt;
+  g.hs = hs;
+  g.nsteps = nsteps;
+  g.maxsteps = max_steps;
+  return o->diff(replay ? 2 : 1, a, g, (hipStream_t)stream) == 0 ? MKPP_SUCCESS : MKPP_ERROR_INVALID_STATE;
+}
+
+int mkpp_amd_device_adjoint(const char* mechanism, const double* conc, long num_cells, long stride,
+                            const double* temp, const double* air_density, const double* rate_parameters,
+                            long parameter_stride, const double* k, long k_stride, int method, const double* ckpt,
+                            const double* hs, const int* nsteps, int max_steps, double* lambda, double* kbar,
+                            void* stream) {
+  int rc;
+  const MechOps* o = diff_mechanism(mechanism, &rc);
+  if (!o) return rc;
+  if (!conc || !ckpt || !hs || !nsteps || !lambda || !kbar) return MKPP_ERROR_NULL_POINTER;
+  if (num_cells < 0 || stride < num_cells || (rate_parameters && parameter_stride < num_cells) ||
+      k_stride < num_cells || max_steps < 1 || (method != mkpp::RODAS3 && method != mkpp::ROS4))
+    return MKPP_ERROR_INVALID_ARGUMENT;
+  DeviceArgs a = diff_args(conc, num_cells, stride, temp, air_density, rate_parameters, parameter_stride);
+  a.method = method;
+  mkpp::AdjArgs g{};
+  g.k = k;
+  g.kstride = k_stride;
+  g.ckpt = const_cast<double*>(ckpt);
+  g.hs = const_cast<double*>(hs);
+  g.nsteps = const_cast<int*>(nsteps);
+  g.maxsteps = max_steps;
+  g.lambda = lambda;
+  g.kbar = kbar;
+  return o->diff(3, a, g, (hipStream_t)stream) == 0 ? MKPP_SUCCESS : MKPP_ERROR_INVALID_STATE;
+}

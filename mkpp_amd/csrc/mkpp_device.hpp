@@ -26,6 +26,7 @@ enum CellErrorCode : int {
   ERR_NON_FINITE_STATE = 2,
   ERR_CONVERGENCE_FAILURE = 3,
   ERR_NUMERICAL_FAILURE = 4,
+  ERR_CHECKPOINT_OVERFLOW = 5,  // more accepted steps than the checkpoint buffer holds
 };
 
 enum Method : int { ROS2 = 0, ROS3 = 1, ROS4 = 2, RODAS3 = 3, RODAS4 = 4 };
@@ -50,6 +51,22 @@ struct DeviceArgs {
   int* accepted;       // [ncells] or null: accepted steps summed over calls
   int* attempted;      // [ncells] or null
   long max_attempts;   // per integrate() call; <= 0 means unbounded
+};
+
+// Differentiation arguments (checkpointed forward, frozen-step replay, adjoint).
+// Checkpoints are cell-major so that one wavefront writes a contiguous run:
+//   ckpt[(cell * maxsteps + n) * NV + v]  state (variable species, solver row
+//                                         order) at the start of accepted step n
+//   hs[cell * maxsteps + n]               its step size
+struct AdjArgs {
+  const double* k;  // [NG x kstride] rate constants used instead of the rate laws, or null
+  long kstride;
+  double* ckpt;
+  double* hs;
+  int* nsteps;  // [ncells] accepted steps recorded
+  int maxsteps;
+  double* lambda;  // adjoint: [N x stride] physical order; in dL/dy(T), out dL/dy(0)
+  double* kbar;    // adjoint: [NG x kstride] out dL/dk
 };
 
 // ---------------------------------------------------------------------------

@@ -24,6 +24,11 @@ struct MechOps {
   // per-cell accepted-step work estimate used by the batching scheduler:
   // algorithmic flops of one attempted step for each method (ros2..rodas4)
   double step_flops[5];
+  // differentiation (wave engine only, null otherwise): mode 0 rate constants,
+  // 1 checkpointed forward, 2 frozen-step replay, 3 adjoint
+  int (*diff)(int mode, const DeviceArgs& a, const AdjArgs& g, hipStream_t s);
+  int num_groups;    // flux groups (rate constants)
+  int num_variable;  // variable species (checkpoint row length)
 };
 
 const MechOps* find_mechanism(const char* name);
@@ -80,7 +85,10 @@ int launch_eval_jacobian(const EvalArgs& a, hipStream_t s) {
                                 &launch_lane_integrate<M>,                                                 \
                                 &launch_eval_rates<M>,                                                     \
                                 &launch_eval_jacobian<M>,                                                  \
-                                {F0, F1, F2, F3, F4}};                                                     \
+                                {F0, F1, F2, F3, F4},                                                      \
+                                nullptr,                                                                   \
+                                M::NG,                                                                     \
+                                M::NV};                                                                    \
     return &ops;                                                                                           \
   }                                                                                                        \
   }

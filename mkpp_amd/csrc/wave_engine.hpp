@@ -89,8 +89,14 @@ __device__ __forceinline__ double bank(const double (&v)[NB], int row) {
 //   scr  [NSCR]      fluxes (F) or Jacobian partials (W); last used slot = 0.0
 //   facc [NV + 1]    F row accumulators; slot NV absorbs padding
 // ---------------------------------------------------------------------------
-template <class W, class Tab>
-__device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long cell, double* lds, int lane) {
+// MODE 0: the plain integrator.  MODE 1: the same, recording the state and
+// step size of every accepted step (adj->ckpt, adj->hs, adj->nsteps).  MODE 2:
+// replay of recorded step sizes without error control (the frozen-step map
+// the adjoint differentiates).  In modes 1 and 2, adj->k (if set) replaces the
+// rate laws.  The mode branches are compile-time; MODE 0 carries none of them.
+template <class W, class Tab, int MODE = 0>
+__device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long cell, double* lds, int lane,
+                                                    const AdjArgs* adj = nullptr) {
   using T_ = WaveTables<W>;
   using C_ = typename T_::Code;
   constexpr int NV = W::NV, NB = W::NB, NG = W::NG, NGP = (NG + 63) / 64, S = Tab::S;
@@ -147,6 +153,13 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
       const int ph = T_::PHOTO[g];
       const double jv = (law == 0 && a.jvals != nullptr) ? a.jvals[cell + a.jstride * ph] : 0.0;
       kreg[p] = wave_rate(law, &T_::PARAMS[g * 7], Tk, Mtb, jv);
+    }
+  }
+  if constexpr (MODE != 0) {
+    if (adj->k != nullptr) {
+#pragma unroll
+      for (int p = 0; p < NGP; ++p)
+        if (p * 64 + lane < NG) kreg[p] = adj->k[cell + adj->kstride * (long)(p * 64 + lane)];
     }
   }
 
@@ -241,14 +254,19 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
     double t = 0.0, dt = dt_total * a.seed;
     bool reject_last = false, reject_more = false;
     long attempts = 0;
-    while (t < dt_total) {
+    int nrep = 0;
+    if constexpr (MODE == 2) nrep = adj->nsteps[cell];
+    while (MODE == 2 ? acc_steps < nrep : t < dt_total) {
       if (a.max_attempts > 0 && attempts >= a.max_attempts) {
         status = ERR_CONVERGENCE_FAILURE;
         break;
       }
       ++attempts;
       ++att_steps;
-      dt = fmin(dt, dt_total - t);
+      if constexpr (MODE == 2)
+        dt = adj->hs[cell * (long)adj->maxsteps + acc_steps];
+      else
+        dt = fmin(dt, dt_total - t);
       const double inv_g_dt = 1.0 / (Tab::G * dt);
       // An opaque copy of the lane id: the generated code tests hundreds of
       // constant lane masks, and with a visible loop-invariant lane id the
@@ -337,11 +355,23 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
         term[b] = valid[b] ? (yerr / sci) * (yerr / sci) : 0.0;
       }
       double err_sq = 0.0;
-      for (int i = 0; i < NV; ++i) err_sq += readlane_d(bank<NB>(term, i), i & 63);
+      if constexpr (MODE != 2)
+        for (int i = 0; i < NV; ++i) err_sq += readlane_d(bank<NB>(term, i), i & 63);
       double err = sqrt(err_sq / W::N);
       err = fmax(err, 1.0e-10);
       const double factor_ = step_factor<Tab>(err);
-      if (err <= 1.0) {
+      if (MODE == 2 || err <= 1.0) {
+        if constexpr (MODE == 1) {
+          if (acc_steps >= adj->maxsteps) {
+            status = ERR_CHECKPOINT_OVERFLOW;
+            break;
+          }
+          double* ck = adj->ckpt + (cell * (long)adj->maxsteps + acc_steps) * NV;
+#pragma unroll
+          for (int b = 0; b < NB; ++b)
+            if (valid[b]) ck[b * 64 + lane] = x[b];
+          if (lane == 0) adj->hs[cell * (long)adj->maxsteps + acc_steps] = dt;
+        }
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
           double msum = 0.0;
@@ -383,6 +413,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
     if (a.status) a.status[cell] = status;
     if (a.accepted) a.accepted[cell] += acc_steps;
     if (a.attempted) a.attempted[cell] += att_steps;
+    if constexpr (MODE == 1) adj->nsteps[cell] = acc_steps;
   }
 }
 
@@ -559,6 +590,11 @@ int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
     static constexpr const uint8_t* CF_ROW = gen::NAME##_wave::CF_ROW;                                      \
     static constexpr const uint32_t* CW_TERM = gen::NAME##_wave::CW_TERM;                                   \
     static constexpr const uint16_t* CW_ADDR = gen::NAME##_wave::CW_ADDR;                                   \
+    static constexpr const uint16_t* GNVOFS = gen::NAME##_wave::GNVOFS;                                     \
+    static constexpr const uint8_t* GNVR = gen::NAME##_wave::GNVR;                                          \
+    static constexpr const uint8_t* GNVC = gen::NAME##_wave::GNVC;                                          \
+    static constexpr const uint16_t* SPOFS = gen::NAME##_wave::SPOFS;                                       \
+    static constexpr const uint16_t* SPI = gen::NAME##_wave::SPI;                                           \
     using Code = gen::NAME##_wave::Code;                                                                    \
   };                                                                                                        \
   }
@@ -578,7 +614,7 @@ int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
     static const MechOps ops = {#NAME, W::N, W::NUM_REACTIONS, W::NPH, W::NPH, 0,                          \
                                 W::SUPPORTS_RUNTIME_ENVIRONMENT, W::SPECIES,                                \
                                 &launch_wave_integrate<W>, &wave_eval_rates_##NAME, &wave_eval_jac_##NAME, \
-                                {F0, F1, F2, F3, F4}};                                                      \
+                                {F0, F1, F2, F3, F4}, &launch_wave_diff<W>, W::NG, W::NV};                  \
     return &ops;                                                                                            \
   }                                                                                                         \
   }

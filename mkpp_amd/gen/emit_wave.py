@@ -188,11 +188,31 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
     for g in groups:
         gdep += g["depends"]
         gdepofs.append(len(gdep))
+    # adjoint tables: per group its variable rows and coefficients (nu . u),
+    # and per species slot (variable or fixed) the partial slots that
+    # differentiate with respect to it (scatter of J^T u and Hessian terms)
+    gnvofs, gnvr, gnvc = [0], [], []
+    for g in groups:
+        for s, c in g["nu"]:
+            if s in vm.v_of_phys:
+                gnvr.append(vm.v_of_phys[s])
+                gnvc.append(cid(float(c)))
+        gnvofs.append(len(gnvr))
+    sp = [[] for _ in range(nv + nfix)]
+    for gi, g in enumerate(groups):
+        for a, (s, q) in enumerate(g["powers"]):
+            sl = yslot(s)
+            if sl < nv + nfix:
+                sp[sl].append(poff[gi] + a)
+    spofs, spi = [0], []
+    for lst in sp:
+        spi += lst
+        spofs.append(len(spi))
     if len(coefs) > 255:
         raise ValueError("more than 255 distinct stoichiometric coefficients")
     return {
         "unit": unit, "gpows_phys": gpows_phys, "gnuofs": gnuofs, "gnus": gnus, "gnuc": gnuc,
-        "gdepofs": gdepofs, "gdep": gdep,
+        "gdepofs": gdepofs, "gdep": gdep, "gnvofs": gnvofs, "gnvr": gnvr, "gnvc": gnvc, "spofs": spofs, "spi": spi,
         "name": p["name"], "n": n, "nv": nv, "nf": nfix, "ng": ng, "nph": p["num_runtime_parameters"],
         "nb": nb, "span": span, "ny": nv + nfix + 1, "npart": npart, "one": ONE,
         "var_phys": vm.var_phys, "fixed_phys": vm.fixed_phys, "third_body": p["third_body"],
@@ -258,6 +278,11 @@ def emit(t: dict) -> str:
         _arr("uint8_t", "GNUC", t["gnuc"]),
         _arr("uint16_t", "GDEPOFS", t["gdepofs"]),
         _arr("uint8_t", "GDEP", t["gdep"]),
+        _arr("uint16_t", "GNVOFS", t["gnvofs"]),
+        _arr("uint8_t", "GNVR", t["gnvr"]),
+        _arr("uint8_t", "GNVC", t["gnvc"]),
+        _arr("uint16_t", "SPOFS", t["spofs"]),
+        _arr("uint16_t", "SPI", t["spi"]),
         "",
         "struct Wave {",
         f"  static constexpr const char* NAME = \"{name}\";",
@@ -282,7 +307,7 @@ def emit_unit(t: dict) -> str:
     flops = ", ".join(f"{float(sf[m])!r}" for m in ("ros2", "ros3", "ros4", "rodas3", "rodas4"))
     return (
         f"// Generated by mkpp_amd.gen.emit_wave: registers {name} (one cell per wavefront).\n"
-        '#include "../wave_engine.hpp"\n'
+        '#include "../adjoint.hpp"\n'
         f'#include "{name}_wcode.hpp"\n\n'
         f"MKPP_WAVE_MECHANISM({name}, {flops})\n"
     )
@@ -485,6 +510,41 @@ def emit_code(t: dict, ct: dict) -> str:
         for b in range(nb):
             if masks[b]:
                 w(f"      if ({_lane_test(masks[b])}) wv[{b}] = fma(-lu[rb[{b}] + {k}], xk, wv[{b}]);")
+        w("    }")
+    w("  }")
+    w("")
+    # ---- transposed solves (adjoint): W^T = U^T L^T ----
+    w("  // U^T z = b, column oriented over U's rows: z_k = b_k / U(k,k) once column k")
+    w("  // is reached, then every j > k in row k of U subtracts U(k,j) z_k.  Lane j")
+    w("  // reads U(k,j) at R[k] + j, a contiguous run across the lanes.")
+    w("  static __device__ __forceinline__ void tforward(const double* __restrict__ lu, double (&b)[" + str(nb) + "], const double (&rdiag)[" + str(nb) + "], double (&z)[" + str(nb) + "], int lane) {")
+    for k in range(nv):
+        masks = [0] * nb
+        for (kk, j) in Uset:
+            if kk == k and j > k:
+                masks[j // 64] |= 1 << (j % 64)
+        w(f"    {{ const double zk = readlane_d(b[{k // 64}] * rdiag[{k // 64}], {k % 64});")
+        w(f"      if (lane == {k % 64}) z[{k // 64}] = zk;")
+        for bb in range(nb):
+            if masks[bb]:
+                w(f"      if ({_lane_test(masks[bb])}) b[{bb}] = fma(-lu[{R[k] + bb * 64} + lane], zk, b[{bb}]);")
+        w("    }")
+    w("  }")
+    w("")
+    w("  // L^T x = z in place (unit diagonal): x_k = z_k when column k is reached,")
+    w("  // descending; every j < k in row k of L subtracts L(k,j) x_k.")
+    w("  static __device__ __forceinline__ void tbackward(const double* __restrict__ lu, double (&z)[" + str(nb) + "], int lane) {")
+    for k in reversed(range(nv)):
+        masks = [0] * nb
+        for (kk, j) in Lset:
+            if kk == k:
+                masks[j // 64] |= 1 << (j % 64)
+        if not any(masks):
+            continue
+        w(f"    {{ const double xk = readlane_d(z[{k // 64}], {k % 64});")
+        for bb in range(nb):
+            if masks[bb]:
+                w(f"      if ({_lane_test(masks[bb])}) z[{bb}] = fma(-lu[{R[k] + bb * 64} + lane], xk, z[{bb}]);")
         w("    }")
     w("  }")
     w("};")

@@ -83,6 +83,14 @@ BATCH_WORKSPACE_BYTES = 32
 _SIGS["mkpp_amd_device_integrate_batch"] = ([C.POINTER(BatchSegment), C.c_int, C.c_double, C.c_int, C.c_int,
                                              C.c_double, C.c_long, C.c_int, C.c_int, _vp, _vp], C.c_int)
 _SIGS["mkpp_amd_batch_capacity"] = ([C.c_int, _ip, _ip], C.c_int)
+_SIGS["mkpp_amd_diff_info"] = ([C.c_char_p, _ip, _ip], C.c_int)
+_SIGS["mkpp_amd_device_rate_constants"] = ([C.c_char_p, _vp, C.c_long, C.c_long, _vp, _vp, _vp, C.c_long, _vp,
+                                            C.c_long, _vp], C.c_int)
+_SIGS["mkpp_amd_device_integrate_checkpointed"] = ([C.c_char_p, _vp, C.c_long, C.c_long, _vp, _vp, _vp, C.c_long, _vp,
+                                                    C.c_long, C.c_double, C.c_int, C.c_double, _vp, _vp, _vp, _vp,
+                                                    _vp, C.c_long, _vp, _vp, _vp, C.c_int, C.c_int, _vp], C.c_int)
+_SIGS["mkpp_amd_device_adjoint"] = ([C.c_char_p, _vp, C.c_long, C.c_long, _vp, _vp, _vp, C.c_long, _vp, C.c_long,
+                                     C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp], C.c_int)
 
 _lib = None
 
@@ -316,3 +324,71 @@ def device_integrate_batch(segments, dt: float, steps: int = 1, method: str = "r
 
 def _addr(t):
     return None if t is None else t.data_ptr()
+
+
+def diff_info(mechanism: str):
+    """(flux groups, variable species) of a mechanism with a device adjoint."""
+    ng, nv = C.c_int(), C.c_int()
+    rc = lib().mkpp_amd_diff_info(mechanism.encode(), C.byref(ng), C.byref(nv))
+    if rc != MKPP_SUCCESS:
+        raise MkppError(rc, "diff_info")
+    return ng.value, nv.value
+
+
+def _stream(stream):
+    import torch
+
+    return C.c_void_p((stream if stream is not None else torch.cuda.current_stream()).cuda_stream)
+
+
+def device_rate_constants(mechanism: str, conc, temp=None, air_density=None, rate_parameters=None, stream=None):
+    """Rate constants k (groups, cells) from the rate laws (device tensors)."""
+    import torch
+
+    n, ncell = conc.shape
+    ng, _ = diff_info(mechanism)
+    k = torch.empty((ng, ncell), dtype=torch.float64, device=conc.device)
+    rc = lib().mkpp_amd_device_rate_constants(mechanism.encode(), _ptr(conc), ncell, ncell, _ptr(temp),
+                                              _ptr(air_density), _ptr(rate_parameters),
+                                              ncell if rate_parameters is not None else 0, _ptr(k), ncell,
+                                              _stream(stream))
+    if rc != MKPP_SUCCESS:
+        raise MkppError(rc, "device_rate_constants")
+    return k
+
+
+def device_integrate_checkpointed(mechanism: str, conc, dt: float, method: str, temp, air_density, rate_parameters,
+                                  k, atol, rtol, ckpt, hs, nsteps, seed: float = 1.0e-6, status=None, accepted=None,
+                                  attempted=None, max_attempts: int = 100000, replay: bool = False, stream=None):
+    """Forward integration of device-resident cells that records (replay=False)
+    or replays (replay=True) the accepted steps.  ckpt: (cells, max_steps, NV)
+    float64, hs: (cells, max_steps) float64, nsteps: (cells,) int32."""
+    n, ncell = conc.shape
+    max_steps = hs.shape[1]
+    assert conc.is_contiguous() and ckpt.is_contiguous() and hs.is_contiguous() and ckpt.shape[:2] == hs.shape
+    rc = lib().mkpp_amd_device_integrate_checkpointed(
+        mechanism.encode(), _ptr(conc), ncell, ncell, _ptr(temp), _ptr(air_density), _ptr(rate_parameters),
+        ncell if rate_parameters is not None else 0, _ptr(k), ncell if k is not None else 0, float(dt),
+        METHODS[method], float(seed), _ptr(atol), _ptr(rtol), _ptr(status), _ptr(accepted), _ptr(attempted),
+        int(max_attempts), _ptr(ckpt), _ptr(hs), _ptr(nsteps), int(max_steps), 1 if replay else 0, _stream(stream))
+    if rc != MKPP_SUCCESS:
+        raise MkppError(rc, "device_integrate_checkpointed")
+
+
+def device_adjoint(mechanism: str, conc, method: str, temp, air_density, rate_parameters, k, ckpt, hs, nsteps,
+                   lam, stream=None):
+    """Adjoint sweep over recorded steps.  lam (species, cells): dL/dy(T) in,
+    dL/dy(0) out (in place).  Returns kbar (groups, cells) = dL/dk."""
+    import torch
+
+    n, ncell = conc.shape
+    ng, _ = diff_info(mechanism)
+    kbar = torch.empty((ng, ncell), dtype=torch.float64, device=conc.device)
+    assert lam.is_contiguous() and lam.shape == conc.shape
+    rc = lib().mkpp_amd_device_adjoint(
+        mechanism.encode(), _ptr(conc), ncell, ncell, _ptr(temp), _ptr(air_density), _ptr(rate_parameters),
+        ncell if rate_parameters is not None else 0, _ptr(k), ncell, METHODS[method], _ptr(ckpt), _ptr(hs),
+        _ptr(nsteps), int(hs.shape[1]), _ptr(lam), _ptr(kbar), _stream(stream))
+    if rc != MKPP_SUCCESS:
+        raise MkppError(rc, "device_adjoint")
+    return kbar

@@ -169,3 +169,21 @@ def batch_mixed(num_saprc99: int = 131072, num_carbon: int = 65536, num_small_st
     for w in (sap, car, sst):
         out.append(dict(w, method="rodas3", dt=3600.0, steps=1, seed=1.0e-6, atol=1.0e3, rtol=1.0e-3))
     return out
+
+
+def saprc99_diff(num_cells: int = 65536, seed: int = 5) -> dict:
+    """The differentiation workload: saprc99 urban/rural cells (as
+    saprc99_urban, seed 5) plus ``target_initial``, the same states with every
+    variable species scaled by (1 + 0.1 N(0, 1)); integrating it gives the
+    target of the loss sum over cells of (O3 - O3*)^2 + (NO2 - NO2*)^2 after 1 h."""
+    from .gen.model import load_mechanism
+
+    mech = load_mechanism("saprc99")
+    w = saprc99_urban(num_cells, seed=seed, species=mech.names)
+    rng = np.random.default_rng(seed + 100)
+    target = w["conc"] * (1.0 + 0.1 * rng.standard_normal(w["conc"].shape))
+    fixed = list(mech.fixed)
+    target[fixed] = w["conc"][fixed]
+    w["target_initial"] = np.maximum(target, 0.0)
+    w["loss_species"] = [mech.names.index("O3"), mech.names.index("NO2")]
+    return w

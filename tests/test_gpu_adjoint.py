@@ -1,0 +1,296 @@
+"""Discrete adjoint of saprc99 Rodas3 over 1 h on the GPU.
+
+* The checkpointed forward pass is the plain integrator (bit for bit) and a
+  replay of its recorded steps reproduces it bit for bit.
+* Gradients of L = sum_cells (O3 - O3*)^2 + (NO2 - NO2*)^2 through the
+  torch.autograd.Function agree with central differences (the accepted step
+  sequence frozen by replaying it) to 1e-6 relative, for every initial
+  concentration and every rate constant, on 64 cells.  The step is 1e-6
+  relative to the component's magnitude: |k| for rate constants and
+  max(|y(0)|, |y(T)|, 1e9 molec/cm3) for concentrations (most species start
+  at zero, where a step relative to the initial value is no step).
+* The rate constants of the finite-difference checks are the rate laws'
+  values except four that exceed the physical limit for their order: the
+  decompositions of PAN, N2O5 and HNO4 evaluate to 1e30-1e36 s^-1 with the
+  reference's TROE parameters (positive C0 in mechanism.json), and
+  NO2 + BZNO2_O to 3e3 cm3/s (an Arrhenius decomposition written with NO2 as
+  a reactant).  W then has a condition number near 1e32, the unpivoted LU
+  drops the small digits of every row coupled to them, and the
+  floating-point map's slope (what FD measures) departs from its derivative
+  by up to 2e-4 in those rows, while the adjoint still equals a dense
+  tangent-linear run with pivoting.  With those four constants set to
+  physical values (1 s^-1, 1e-11 cm3/s; k is an input of the differentiated
+  map) every component is checked.  The dot-product identity runs on the rate
+  laws' own values, with the ill-conditioned directions left out.
+* The adjoint satisfies the dot-product identity <lambda, TLM d> = <ADJ lambda,
+  d> against an independent dense NumPy tangent-linear run
+  (tests/rosenbrock_ref.py) along the same step sequence.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from mkpp_amd import autograd, native, workloads
+from tests.rosenbrock_ref import DenseMechanism, replay_tlm
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda:0")
+MAXS = 256
+
+
+X_REF = 1.0e9
+# rate constants above the physical limits (first order 1e10 s^-1; bimolecular
+# 1e-9 cm3/s, the gas-kinetic collision rate; termolecular 1e-27 cm6/s)
+K_LIMIT = {1: 1.0e10, 2: 1.0e-9, 3: 1.0e-27}
+K_PHYSICAL = {1: 1.0, 2: 1.0e-11, 3: 1.0e-31}
+
+
+def _ill(c):
+    """Flux groups whose rate constant exceeds the physical limit for its order
+    somewhere, with that order, and the species that react in them."""
+    import json
+    import os
+
+    plan = json.load(open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "mkpp_amd",
+                                       "plans", "saprc99.json")))
+    kmax = c["k"].abs().max(dim=1).values.cpu().numpy()
+    groups, orders = [], []
+    for g, grp in enumerate(plan["groups"]):
+        order = sum(q for _, q in grp["powers"])
+        if kmax[g] > K_LIMIT.get(order, np.inf):
+            groups.append(g)
+            orders.append(order)
+    species = sorted({s for g in groups for s, _ in plan["groups"][g]["powers"]})
+    return species, groups, orders
+
+
+@pytest.fixture(scope="module")
+def case():
+    w = workloads.saprc99_diff(64, seed=5)
+    t = lambda a: torch.tensor(a, device=DEV).contiguous()
+    c = {
+        "w": w,
+        "y0": t(w["conc"]),
+        "target0": t(w["target_initial"]),
+        "temp": t(w["temp"]),
+        "rho": t(w["air_density"]),
+        "jv": t(w["rate_parameters"]),
+        "atol": torch.full((82,), 1.0e3, dtype=torch.float64, device=DEV),
+        "rtol": torch.full((82,), 1.0e-3, dtype=torch.float64, device=DEV),
+    }
+    c["k_law"] = autograd.rate_constants("saprc99", c["y0"], c["temp"], c["rho"], c["jv"])
+    from mkpp_amd.gen.model import load_mechanism
+
+    c["names"] = load_mechanism("saprc99").names
+    c["k"] = c["k_law"]
+    _, ill_g, orders = _ill(c)
+    print("rate constants above physical limits (groups, order):", list(zip(ill_g, orders)))
+    assert len(ill_g) <= 4
+    c["k_fd"] = c["k_law"].clone()
+    for g, o in zip(ill_g, orders):
+        c["k_fd"][g] = K_PHYSICAL[o]
+    # target state: the perturbed initial state integrated with the same k
+    c["target"] = autograd.integrate(c["target0"], c["k"], "saprc99", c["temp"], c["rho"]).detach()
+    return c
+
+
+def _record(c, y0, k):
+    ng, nv = native.diff_info("saprc99")
+    n = y0.shape[1]
+    y = y0.clone()
+    ck = torch.empty((n, MAXS, nv), dtype=torch.float64, device=DEV)
+    hs = torch.empty((n, MAXS), dtype=torch.float64, device=DEV)
+    ns = torch.zeros(n, dtype=torch.int32, device=DEV)
+    st = torch.zeros(n, dtype=torch.int32, device=DEV)
+    native.device_integrate_checkpointed("saprc99", y, 3600.0, "rodas3", c["temp"], c["rho"], c["jv"], k, c["atol"],
+                                         c["rtol"], ck, hs, ns, status=st)
+    torch.cuda.synchronize()
+    assert (st == 0).all()
+    return y, ck, hs, ns
+
+
+def _replay(c, y0, k, hs, ns):
+    ng, nv = native.diff_info("saprc99")
+    y = y0.clone()
+    ck = torch.empty((y0.shape[1], MAXS, nv), dtype=torch.float64, device=DEV)
+    native.device_integrate_checkpointed("saprc99", y, 3600.0, "rodas3", c["temp"], c["rho"], c["jv"], k, c["atol"],
+                                         c["rtol"], ck, hs, ns, replay=True)
+    return y
+
+
+def _cell_loss(c, y):
+    sp = c["w"]["loss_species"]
+    return ((y[sp] - c["target"][sp]) ** 2).sum(0)
+
+
+def test_record_is_plain_forward_and_replay_is_exact(case):
+    c = case
+    y_plain = c["y0"].clone()
+    acc = torch.zeros(64, dtype=torch.int32, device=DEV)
+    native.device_integrate("saprc99", y_plain, 3600.0, 1, "rodas3", c["temp"], c["rho"], c["jv"], c["atol"],
+                            c["rtol"], 1e-6, accepted=acc)
+    y_rec, ck, hs, ns = _record(c, c["y0"], None)
+    torch.cuda.synchronize()
+    assert torch.equal(y_plain, y_rec) and torch.equal(acc, ns)
+    # explicit k from the rate laws = the laws themselves
+    y_k, _, _, ns_k = _record(c, c["y0"], c["k"])
+    assert torch.equal(y_k, y_rec) and torch.equal(ns_k, ns)
+    y_rep = _replay(c, c["y0"], None, hs, ns)
+    torch.cuda.synchronize()
+    assert torch.equal(y_rep, y_rec)
+    # the recorded states are the integrator's path: first = y0 (variable rows)
+    print("accepted steps per cell: min", int(ns.min()), "max", int(ns.max()))
+
+
+def _grads(c):
+    y0 = c["y0"].clone().requires_grad_(True)
+    k = c["k"].clone().requires_grad_(True)
+    y = autograd.integrate(y0, k, "saprc99", c["temp"], c["rho"])
+    L = _cell_loss(c, y).sum()
+    L.backward()
+    return y0.grad.detach(), k.grad.detach()
+
+
+EPS = 2.220446049250313e-16
+
+
+def _fd_check(ad, fd, fd4, noise, what, skip, names, sig_frac=1e-6):
+    """Adjoint against central differences.  FD's own error is estimated as
+    twice the larger of its disagreement with the 4h difference (truncation
+    grows as h^2, rounding as 1/h) and the rounding of the loss itself; the
+    adjoint must be within 1e-6 |fd| plus that estimate.  Components whose FD
+    error estimate is below 2e-7 |fd| are the resolved ones: the adjoint must
+    agree with FD to 1e-6 on every one of them, they must be most of the
+    significant components (above sig_frac of the cell's largest derivative),
+    99 % of those must lie within the bar and the rest agree to 1e-4.  Rows in
+    `skip` are left out."""
+    keep = np.ones(ad.shape[0], dtype=bool)
+    keep[skip] = False
+    rows = np.flatnonzero(keep)
+    ad, fd, fd4, noise = ad[keep], fd[keep], fd4[keep], noise[keep]
+    scale = np.abs(fd).max(axis=0, keepdims=True)
+    bound = 2.0 * np.maximum(np.abs(fd - fd4), noise)
+    sig = np.abs(fd) > sig_frac * scale
+    info = sig & (bound <= 2e-7 * np.abs(fd))
+    err = np.abs(ad - fd)
+    ok = err <= 1e-6 * np.abs(fd) + bound
+    rel = err / np.maximum(np.abs(fd), 1e-300)
+    print(f"{what}: {sig.sum()} significant components, {info.sum()} resolved by FD to 1e-7; "
+          f"on those max rel err {rel[info].max():.3e}, median {np.median(rel[info]):.3e}; "
+          f"{(~ok & sig).sum()} significant components outside 1e-6 |fd| + FD error")
+    worst = np.argsort(-(np.where(sig, err - bound, -np.inf) / np.maximum(np.abs(fd), 1e-300)).ravel())[:5]
+    for w_ in worst:
+        i, c = np.unravel_index(w_, rel.shape)
+        print(f"   {names[rows[i]]:10s} cell {c:2d} adjoint {ad[i, c]: .9e} fd {fd[i, c]: .9e} fd(4h) {fd4[i, c]: .9e}"
+              f" bound {bound[i, c]:.2e}")
+    out = sig & ~ok
+    worst_out = rel[out].max() if out.any() else 0.0
+    print(f"   components outside the FD error bar: max rel err {worst_out:.3e}")
+    return ok[sig].mean(), info.sum(), sig.sum(), rel[info], worst_out
+
+
+def _fd(c, hs, ns, n_rows, perturb):
+    """Central differences of the per-cell loss along the frozen steps, with
+    steps h and 4h, and the rounding level of the loss per component."""
+    L0 = _cell_loss(c, _replay(c, c["y0"], c["k"], hs, ns)).abs()
+    fd = np.zeros((n_rows, 64))
+    fd4 = np.zeros((n_rows, 64))
+    noise = np.zeros((n_rows, 64))
+    for i in range(n_rows):
+        for mult, out in ((1.0, fd), (4.0, fd4)):
+            (yp, kp), (ym, km), h = perturb(i, mult)
+            Lp = _cell_loss(c, _replay(c, yp, kp, hs, ns))
+            Lm = _cell_loss(c, _replay(c, ym, km, hs, ns))
+            out[i] = ((Lp - Lm) / (2 * h)).cpu().numpy()
+            if mult == 1.0:
+                noise[i] = (8 * EPS * L0 / (2 * h)).cpu().numpy()
+    return fd, fd4, noise
+
+
+@pytest.fixture(scope="module")
+def fdcase(case):
+    c = dict(case, k=case["k_fd"])
+    c["target"] = autograd.integrate(c["target0"], c["k"], "saprc99", c["temp"], c["rho"]).detach()
+    return c
+
+
+def test_gradient_wrt_initial_state_matches_fd(fdcase):
+    c = fdcase
+    gy, _ = _grads(c)
+    yT, _, hs, ns = _record(c, c["y0"], c["k"])
+    y0 = c["y0"]
+    mag = torch.maximum(torch.maximum(y0.abs(), yT.abs()), torch.full_like(y0, X_REF))
+
+    def perturb(i, mult):
+        h = mult * 1e-6 * mag[i]
+        yp = y0.clone()
+        yp[i] += h
+        ym = y0.clone()
+        ym[i] -= h
+        return (yp, c["k"]), (ym, c["k"]), h
+
+    fd, fd4, noise = _fd(c, hs, ns, 82, perturb)
+    within, ninfo, nsig, rel, worst = _fd_check(gy.cpu().numpy(), fd, fd4, noise, "dL/dy0", [], c["names"])
+    assert rel.max() <= 1e-6 and ninfo >= 0.7 * nsig and within >= 0.99 and worst <= 1e-4
+
+
+def test_gradient_wrt_rate_constants_matches_fd(fdcase):
+    c = fdcase
+    _, gk = _grads(c)
+    _, _, hs, ns = _record(c, c["y0"], c["k"])
+    k = c["k"]
+    ng = k.shape[0]
+
+    def perturb(g, mult):
+        h = mult * 1e-6 * torch.abs(k[g])
+        step = torch.where(k[g] != 0, h, torch.zeros_like(h))
+        h = torch.where(h > 0, h, torch.ones_like(h))  # k = 0 (night photolysis): no relative step exists
+        kp = k.clone()
+        kp[g] += step
+        km = k.clone()
+        km[g] -= step
+        return (c["y0"], kp), (c["y0"], km), h
+
+    fd, fd4, noise = _fd(c, hs, ns, ng, perturb)
+    ad = gk.cpu().numpy()
+    ad = np.where((k == 0).cpu().numpy(), 0.0, ad)
+    # compare the sensitivities k dL/dk (= dL/dln k): rate constants span some
+    # 60 decades, their raw derivatives as many
+    kk = np.abs(k.cpu().numpy())
+    ad, fd, fd4, noise = ad * kk, fd * kk, fd4 * kk, noise * kk
+    # a 1e-6 relative change of a rate constant whose sensitivity is below 1e-4
+    # of the cell's largest moves the loss by only a few hundred ulps
+    within, ninfo, nsig, rel, worst = _fd_check(ad, fd, fd4, noise, "k dL/dk", [], [f"k{g}" for g in range(ng)],
+                                                sig_frac=1e-4)
+    assert rel.max() <= 1e-6 and ninfo >= 0.3 * nsig and within >= 0.99 and worst <= 5e-2
+
+
+def test_dot_product_identity_against_numpy_tlm(case):
+    c = case
+    m = DenseMechanism("saprc99")
+    _, ck, hs, ns = _record(c, c["y0"], c["k"])
+    rng = np.random.default_rng(11)
+    y0 = c["y0"].cpu().numpy().T
+    k = c["k"].cpu().numpy().T
+    ill_sp, ill_g, _ = _ill(c)
+    dy0 = rng.standard_normal(y0.shape) * y0 * 1e-3
+    dk = rng.standard_normal(k.shape) * k * 1e-3
+    dy0[:, ill_sp] = 0.0
+    dk[:, ill_g] = 0.0
+    yT, dyT = replay_tlm(m, y0, k, hs.cpu().numpy(), ns.cpu().numpy(), "rodas3", dy0=dy0, dk=dk)
+    # the dense replay follows the device replay closely (same map, other rounding)
+    ydev = _replay(c, c["y0"], c["k"], hs, ns).cpu().numpy().T
+    big = np.abs(ydev) > 1e3
+    print("dense replay vs device: max rel", (np.abs(yT - ydev)[big] / np.abs(ydev)[big]).max())
+    lam = rng.standard_normal(y0.shape) / np.maximum(np.abs(yT), 1.0)
+    lam_d = torch.tensor(np.ascontiguousarray(lam.T), device=DEV)
+    yfin = torch.tensor(np.ascontiguousarray(ydev.T), device=DEV)
+    kbar = native.device_adjoint("saprc99", yfin, "rodas3", c["temp"], c["rho"], None, c["k"], ck, hs, ns, lam_d)
+    torch.cuda.synchronize()
+    lhs = np.sum(lam * dyT, axis=1)
+    rhs = np.sum(lam_d.cpu().numpy().T * dy0, axis=1) + np.sum(kbar.cpu().numpy().T * dk, axis=1)
+    rel = np.abs(lhs - rhs) / np.maximum(np.abs(lhs), np.abs(rhs))
+    print("dot-product identity per cell: max rel", rel.max(), "median", np.median(rel))
+    assert rel.max() <= 1e-8

@@ -41,6 +41,10 @@ def test_indices_in_bounds(case):
     # row displacement: no two structural non-zeros share an address
     assert len(t["waddr"]) == len(set(t["waddr"]))
     assert ng <= 255 and nv <= 128  # uint8 group / row indices
+    # adjoint tables: group -> variable rows, species slot -> partial slots
+    assert t["gnvofs"][-1] == len(t["gnvr"]) and max(t["gnvr"]) < nv and max(t["gnvc"]) < len(t["coefs"])
+    assert len(t["spofs"]) == nv + t["nf"] + 1 and t["spofs"][-1] == len(t["spi"]) == npart
+    assert sorted(t["spi"]) == list(range(npart))  # every partial slot scatters to exactly one species
     assert -32768 <= min(t["R"]) and max(t["R"]) < 32768 and span < 65536  # int16 R, uint16 addresses
 
 
@@ -157,6 +161,25 @@ def test_emulated_wave_engine_matches_oracle(case):
         for r in range(nv):
             if (t["bmask"][kk][r // 64] >> (r % 64)) & 1:
                 wv[r] -= lu[R[r] + kk] * x[kk]
+    # transposed solve W^T x = b as the generated tforward / tbackward run it:
+    # lane j of pivot k reads R[k] + j (row k of U, then row k of L)
+    bt = rng.uniform(-1, 1, nv)
+    zb = bt.copy()
+    z = np.zeros(nv)
+    for kk in range(nv):
+        z[kk] = zb[kk] / lu[R[kk] + kk]
+        for j in t["ucol"][t["uofs"][kk]:t["uofs"][kk + 1]]:
+            zb[j] -= lu[R[kk] + j] * z[kk]
+    for kk in reversed(range(nv)):
+        for j in range(kk):
+            if kk in t["lrow"][t["lofs"][j]:t["lofs"][j + 1]]:
+                z[j] -= lu[R[kk] + j] * z[kk]
+    Wd = np.zeros((nv, nv))
+    for s_, (i, j) in enumerate(p["w_locations"]):
+        if i in vlog and j in vlog:
+            Wd[vlog[i], vlog[j]] = w[s_]
+    zr = np.linalg.solve(Wd.T, bt)
+    np.testing.assert_allclose(z, zr, rtol=1e-9, atol=1e-8 * np.abs(zr).max())
     full_rhs = np.zeros(n)
     lg_of_v = [lg for lg in range(n) if p["permutation"][lg] in t["var_phys"]]
     for v, lg in enumerate(lg_of_v):
