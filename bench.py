@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-batch", dest="batch", action="store_false",
                     help="skip the mixed-mechanism persistent-vs-plain comparison reported as batching")
+    ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
+                    help="skip the differentiation measurement reported as adjoint")
     ap.add_argument("--no-pcie", dest="pcie", action="store_false",
                     help="skip the host-buffer (handle API) run that is reported as pcie_inclusive")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -129,6 +131,7 @@ def main():
                 "what": "mkpp_integrate on host LayoutLeft buffers: host-to-device staging, kernel, copy back"}
 
     batching = run_batching(np) if args.batch else None
+    adjoint = run_adjoint(np) if args.adjoint else None
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "saprc99_rodas3_262144_traffic.json")
@@ -177,10 +180,81 @@ def main():
         "cpu_baseline": cpu,
         "pcie_inclusive": pcie,
         "batching": batching,
+        "adjoint": adjoint,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
     print(json.dumps(out))
+
+
+def run_adjoint(np, cells=65536, max_steps=128):
+    """The differentiation workload: saprc99 Rodas3 1 h over 65536 cells.  Times
+    the plain forward launch and the checkpointed forward plus the adjoint
+    sweep (gradients of the O3/NO2 loss w.r.t. y0 and the 210 rate constants);
+    the ratio is the second over the first.  Fresh state for every timed run."""
+    import torch
+
+    from mkpp_amd import native, workloads
+
+    dev = torch.device("cuda:0")
+    w = workloads.saprc99_diff(cells)
+    t = lambda a: torch.tensor(a, device=dev).contiguous()
+    y0, T, rho, jv = t(w["conc"]), t(w["temp"]), t(w["air_density"]), t(w["rate_parameters"])
+    n = y0.shape[0]
+    at = torch.full((n,), 1.0e3, dtype=torch.float64, device=dev)
+    rt = torch.full((n,), 1.0e-3, dtype=torch.float64, device=dev)
+    k = native.device_rate_constants("saprc99", y0, T, rho, jv)
+    ng, nv = native.diff_info("saprc99")
+    ck = torch.empty((cells, max_steps, nv), dtype=torch.float64, device=dev)
+    hs = torch.empty((cells, max_steps), dtype=torch.float64, device=dev)
+    ns = torch.zeros(cells, dtype=torch.int32, device=dev)
+    st = torch.zeros(cells, dtype=torch.int32, device=dev)
+    # target: the perturbed initial state integrated with the same rate constants
+    tgt = t(w["target_initial"])
+    native.device_integrate_checkpointed("saprc99", tgt, 3600.0, "rodas3", T, rho, jv, k, at, rt, ck, hs, ns)
+    sp = w["loss_species"]
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, reps=2):
+        ms = []
+        for _ in range(reps):
+            y = y0.clone()
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            out = fn(y)
+            b.record(stream)
+            torch.cuda.synchronize()
+            ms.append(a.elapsed_time(b))
+        return float(np.mean(ms)), out
+
+    def plain(y):
+        native.device_integrate("saprc99", y, 3600.0, 1, "rodas3", T, rho, jv, at, rt, 1.0e-6, status=st)
+
+    def fwd_adj(y):
+        native.device_integrate_checkpointed("saprc99", y, 3600.0, "rodas3", T, rho, jv, k, at, rt, ck, hs, ns,
+                                             status=st)
+        lam = torch.zeros_like(y)
+        lam[sp] = 2.0 * (y[sp] - tgt[sp])
+        kbar = native.device_adjoint("saprc99", y, "rodas3", T, rho, jv, k, ck, hs, ns, lam)
+        return lam, kbar
+
+    def fwd_ckpt(y):
+        native.device_integrate_checkpointed("saprc99", y, 3600.0, "rodas3", T, rho, jv, k, at, rt, ck, hs, ns,
+                                             status=st)
+
+    plain(y0.clone())
+    fwd_adj(y0.clone())
+    plain_ms, _ = timed(plain)
+    ckpt_ms, _ = timed(fwd_ckpt)
+    total_ms, (lam, kbar) = timed(fwd_adj)
+    bad = int((st != 0).sum())
+    return {"workload": f"saprc99 Rodas3 1 h, {cells} cells, loss sum (O3 - O3*)^2 + (NO2 - NO2*)^2, "
+                        "gradients w.r.t. y0 (82) and 210 rate constants per cell",
+            "forward_ms": plain_ms, "checkpointed_forward_ms": ckpt_ms, "forward_plus_adjoint_ms": total_ms,
+            "adjoint_sweep_ms": total_ms - ckpt_ms, "adjoint_over_forward": total_ms / plain_ms,
+            "accepted_steps_mean": float(ns.float().mean()), "checkpoint_bytes": ck.numel() * 8 + hs.numel() * 8,
+            "nonzero_status_cells": bad, "grad_finite": bool(torch.isfinite(lam).all() and torch.isfinite(kbar).all())}
 
 
 def run_batching(np):

@@ -35,7 +35,7 @@ namespace mkpp {
 // stage, the u vector of a J^T pass and the fixed-species adjoint.
 template <class W, class Tab>
 constexpr int adjoint_lds_doubles() {
-  return W::LDS_DOUBLES + 2 * Tab::S * (W::NV + 1) + Tab::S * W::NY + (W::NV + 1) + (W::NF + 1);
+  return W::LDS_DOUBLES + 2 * Tab::S * (W::NV + 1) + Tab::S * W::NY + (W::NV + 1) + (W::NF + 1) + W::NY;
 }
 
 template <class W, class Tab>
@@ -53,6 +53,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
   double* kbv = kv + S * NY;         // [S][NV+1] Kbar_s by row
   double* uv = kbv + S * (NV + 1);   // [NV+1]    u of a J^T pass by row
   double* fbar = uv + (NV + 1);      // [NF]      adjoint of the fixed species
+  double* sacc = fbar + (NF + 1);    // [NY]      scatter sums per species slot
 
   int rb[NB];
   bool valid[NB];
@@ -198,29 +199,55 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
     return w;
   };
 
-  // scr[partial slot] summed per species slot: rows into out[], fixed species into fbar
+  // scr[partial slot] summed per species slot: rows into out[], fixed species
+  // into fbar.  Fixed-trip chunks (CS tables): one chunk of partial slots per
+  // lane and pass, the chunks of one species in successive passes.
   auto scatter = [&](double (&out)[NB]) {
+    if (lane == 0) scr[W::NPART] = 0.0;
+    for (int i = lane; i < NV + NF; i += 64) sacc[i] = 0.0;
+#pragma unroll
+    for (int p = 0; p < C_::NSP; ++p) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < C_::TS; ++q) acc += scr[T_::CS_TERM[(p * C_::TS + q) * 64 + lane]];
+      const int row = T_::CS_ROW[p * 64 + lane];
+      if (row != 255) sacc[row] += acc;
+    }
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int l = b * 64 + lane;
-      double acc = 0.0;
-      if (l < NV + NF)
-        for (int t = T_::SPOFS[l]; t < T_::SPOFS[l + 1]; ++t) acc += scr[T_::SPI[t]];
-      out[b] = valid[b] ? acc : 0.0;
-      if (l >= NV && l < NV + NF) fbar[l - NV] += acc;
+      out[b] = valid[b] ? sacc[l] : 0.0;
+      if (l >= NV && l < NV + NF) fbar[l - NV] += sacc[l];
     }
   };
 
   // J(Y)^T u at the state in ylds (u by row in uv), plus kbar += u . dF/dk
-  auto jt_pass = [&](double (&out)[NB]) {
+  // The group-block loops of the two passes stay rolled (four copies of the
+  // second-derivative code would not fit the register file); the block's rate
+  // constant and kbar accumulator are selected by the runtime block index.
+  auto kreg_at = [&](int p) {
+    double v = 0.0;
 #pragma unroll
+    for (int q = 0; q < NGP; ++q)
+      if (q == p) v = kreg[q];
+    return v;
+  };
+  auto kbar_add = [&](int p, double v) {
+#pragma unroll
+    for (int q = 0; q < NGP; ++q)
+      if (q == p) kbar[q] += v;
+  };
+
+  auto jt_pass = [&](double (&out)[NB]) {
+#pragma unroll 1
     for (int p = 0; p < NGP; ++p) {
       const int gi = p * 64 + lane;
       if (gi < NG) {
         const double w = nu_dot(gi, uv);
         double pw[3], pm1[3], pm2[3];
         monomial(gi, pw, pm1, pm2);
-        kbar[p] = fma(w, (pw[0] * pw[1]) * pw[2], kbar[p]);
+        kbar_add(p, w * ((pw[0] * pw[1]) * pw[2]));
+        const double kp = kreg_at(p);
         const int np = T_::GNPOW[gi];
         const int off = T_::POFF[gi];
 #pragma unroll
@@ -229,7 +256,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
 #pragma unroll
           for (int o = 0; o < 3; ++o)
             if (o != e) d *= pw[o];
-          if (e < np) scr[off + e] = (w * kreg[p]) * d;
+          if (e < np) scr[off + e] = (w * kp) * d;
         }
       }
     }
@@ -238,7 +265,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
 
   // second-derivative terms at y_n: sum_s d/dy [rbar_s^T J K_s] and d/dk
   auto hess_pass = [&](double (&out)[NB]) {
-#pragma unroll
+#pragma unroll 1
     for (int p = 0; p < NGP; ++p) {
       const int gi = p * 64 + lane;
       if (gi < NG) {
@@ -271,10 +298,21 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
             d2[e][f] = h;
           }
         }
+        // nu_g . rbar_s for every stage in one pass over the group's rows
+        double ws[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) ws[s] = 0.0;
+        for (int t = T_::GNVOFS[gi]; t < T_::GNVOFS[gi + 1]; ++t) {
+          const double cf = T_::COEF[T_::GNVC[t]];
+          const int r = T_::GNVR[t];
+#pragma unroll
+          for (int s = 0; s < S; ++s) ws[s] = fma(cf, rbv[s * (NV + 1) + r], ws[s]);
+        }
         double val[3] = {0.0, 0.0, 0.0};
         double kb = 0.0;
+#pragma unroll
         for (int s = 0; s < S; ++s) {
-          const double w = nu_dot(gi, rbv + s * (NV + 1));
+          const double w = ws[s];
           double kvs[3];
 #pragma unroll
           for (int e = 0; e < 3; ++e) kvs[e] = kv[s * NY + sl[e]];
@@ -289,12 +327,13 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
           }
           kb = fma(w, dk, kb);
         }
-        kbar[p] += kb;
+        kbar_add(p, kb);
+        const double kp = kreg_at(p);
         const int np = T_::GNPOW[gi];
         const int off = T_::POFF[gi];
 #pragma unroll
         for (int e = 0; e < 3; ++e)
-          if (e < np) scr[off + e] = kreg[p] * val[e];
+          if (e < np) scr[off + e] = kp * val[e];
       }
     }
     scatter(out);
@@ -461,7 +500,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
 }
 
 template <class W, class Tab>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) wave_adjoint_kernel(DeviceArgs a, AdjArgs g) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) wave_adjoint_kernel(DeviceArgs a, AdjArgs g) {
   extern __shared__ double lds[];
   const long first = (long)(blockIdx.x % 8u) * (gridDim.x / 8u) + blockIdx.x / 8u;
   for (long cell = first; cell < a.ncells; cell += gridDim.x) wave_adjoint_cell<W, Tab>(a, g, cell, lds, threadIdx.x);

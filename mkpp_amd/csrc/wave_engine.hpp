@@ -595,6 +595,8 @@ int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
     static constexpr const uint8_t* GNVC = gen::NAME##_wave::GNVC;                                          \
     static constexpr const uint16_t* SPOFS = gen::NAME##_wave::SPOFS;                                       \
     static constexpr const uint16_t* SPI = gen::NAME##_wave::SPI;                                           \
+    static constexpr const uint16_t* CS_TERM = gen::NAME##_wave::CS_TERM;                                   \
+    static constexpr const uint8_t* CS_ROW = gen::NAME##_wave::CS_ROW;                                      \
     using Code = gen::NAME##_wave::Code;                                                                    \
   };                                                                                                        \
   }

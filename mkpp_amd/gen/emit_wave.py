@@ -416,8 +416,24 @@ def build_code_tables(t: dict, tf: int = 4, tw: int = 2) -> dict:
 
     fterm, frow = lay(fpasses, tf, ZF, 255)
     wterm, waddr = lay(wpasses, tw, ZW, 0xFFFF)
+    # adjoint scatter: chunks of partial slots per species slot (variable and
+    # fixed), padded with the zero partial slot
+    ts = 4
+    sitems = []
+    for l in range(t["nv"] + t["nf"]):
+        terms = list(t["spi"][t["spofs"][l]:t["spofs"][l + 1]])
+        if terms:
+            sitems.append((l, terms))
+    spasses = _pack_chunks(sitems, ts)
+    sterm, srow = [], []
+    for ps in spasses:
+        for q in range(ts):
+            for lane in range(64):
+                sterm.append(ps[lane][1][q] if lane < len(ps) and q < len(ps[lane][1]) else ZW)
+        for lane in range(64):
+            srow.append(ps[lane][0] if lane < len(ps) else 255)
     return {"tf": tf, "tw": tw, "nfp": len(fpasses), "nwp": len(wpasses), "fterm": fterm, "frow": frow,
-            "wterm": wterm, "wkey": waddr}
+            "wterm": wterm, "wkey": waddr, "ts": ts, "nsp": len(spasses), "sterm": sterm, "srow": srow}
 
 
 def emit_code(t: dict, ct: dict) -> str:
@@ -444,9 +460,12 @@ def emit_code(t: dict, ct: dict) -> str:
     w(_arr("uint8_t", "CF_ROW", ct["frow"]))
     w(_arr("uint32_t", "CW_TERM", ct["wterm"]))
     w(_arr("uint16_t", "CW_ADDR", ct["wkey"]))
+    w(_arr("uint16_t", "CS_TERM", ct["sterm"]))
+    w(_arr("uint8_t", "CS_ROW", ct["srow"]))
     w("")
     w("struct Code {")
     w(f"  static constexpr int TF = {ct['tf']}, NFP = {ct['nfp']}, TW = {ct['tw']}, NWP = {ct['nwp']};")
+    w(f"  static constexpr int TS = {ct['ts']}, NSP = {ct['nsp']};  // adjoint scatter chunks")
     w("")
     # ---- factorization ----
     w("  // Right-looking factorization: pivot k divides column k of every row in")
