@@ -55,12 +55,14 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
   double* fbar = uv + (NV + 1);      // [NF]      adjoint of the fixed species
   double* sacc = fbar + (NF + 1);    // [NY]      scatter sums per species slot
 
-  int rb[NB];
+  // lane slot (b, lane) holds row ROWOF[b * 64 + lane] (255: none)
+  int rb[NB], rowv[NB];
   bool valid[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    const int row = b * 64 + lane;
-    valid[b] = row < NV;
+    const int row = T_::ROWOF[b * 64 + lane];
+    valid[b] = row != 255;
+    rowv[b] = valid[b] ? row : 0;
     rb[b] = valid[b] ? (int)T_::R[row] : 0;
   }
   const int nst = g.nsteps[cell];
@@ -92,17 +94,14 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
   // ---- lambda = dL/dy(T) ----
   double lam[NB];
 #pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    const int row = b * 64 + lane;
-    lam[b] = valid[b] ? g.lambda[cell + a.stride * (long)T_::VARPHYS[row]] : 0.0;
-  }
+  for (int b = 0; b < NB; ++b) lam[b] = valid[b] ? g.lambda[cell + a.stride * (long)T_::VARPHYS[rowv[b]]] : 0.0;
   if (lane < NF) fbar[lane] = g.lambda[cell + a.stride * (long)T_::FIXPHYS[lane]];
   for (int i = lane; i < S * NY; i += 64) kv[i] = 0.0;
 
   auto store_y = [&](const double (&y)[NB]) {
 #pragma unroll
     for (int b = 0; b < NB; ++b)
-      if (valid[b]) ylds[b * 64 + lane] = y[b];
+      if (valid[b]) ylds[rowv[b]] = y[b];
   };
 
   auto eval_rates = [&](double (&f)[NB]) {
@@ -128,7 +127,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
       if (row != 255) facc[row] = facc[row] + acc;
     }
 #pragma unroll
-    for (int b = 0; b < NB; ++b) f[b] = valid[b] ? facc[b * 64 + lane] : 0.0;
+    for (int b = 0; b < NB; ++b) f[b] = valid[b] ? facc[rowv[b]] : 0.0;
   };
 
   // reactant powers of group gi at the state in ylds: pw = y^q, pm1 = q y^(q-1),
@@ -178,7 +177,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
     for (int i = lane; i <= W::SPAN; i += 64) lu[i] = 0.0;
 #pragma unroll
     for (int b = 0; b < NB; ++b)
-      if (valid[b]) lu[rb[b] + b * 64 + lane] = inv_g_dt;
+      if (valid[b]) lu[rb[b] + rowv[b]] = inv_g_dt;
 #pragma unroll
     for (int p = 0; p < C_::NWP; ++p) {
       double acc = 0.0;
@@ -214,11 +213,8 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
       if (row != 255) sacc[row] += acc;
     }
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      const int l = b * 64 + lane;
-      out[b] = valid[b] ? sacc[l] : 0.0;
-      if (l >= NV && l < NV + NF) fbar[l - NV] += sacc[l];
-    }
+    for (int b = 0; b < NB; ++b) out[b] = valid[b] ? sacc[rowv[b]] : 0.0;
+    if (lane < NF) fbar[lane] += sacc[NV + lane];
   };
 
   // J(Y)^T u at the state in ylds (u by row in uv), plus kbar += u . dF/dk
@@ -344,7 +340,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
     const double* ck = g.ckpt + (cell * (long)g.maxsteps + n) * NV;
     double x[NB];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) x[b] = valid[b] ? ck[b * 64 + lane] : 0.0;
+    for (int b = 0; b < NB; ++b) x[b] = valid[b] ? ck[rowv[b]] : 0.0;
     const double inv_g_dt = 1.0 / (Tab::G * dt);
     int ln = lane;
     asm volatile("" : "+v"(ln));
@@ -353,7 +349,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
     assemble_w(inv_g_dt);
     C_::factor(lu, rb, ln);
 #pragma unroll
-    for (int b = 0; b < NB; ++b) rdiag[b] = valid[b] ? 1.0 / lu[rb[b] + b * 64 + lane] : 0.0;
+    for (int b = 0; b < NB; ++b) rdiag[b] = valid[b] ? 1.0 / lu[rb[b] + rowv[b]] : 0.0;
 
     // ---- forward stages, recomputed from the checkpoint ----
 #pragma unroll 1
@@ -406,7 +402,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
     for (int s = 0; s < S; ++s)
 #pragma unroll
       for (int b = 0; b < NB; ++b)
-        if (valid[b]) kv[s * NY + b * 64 + lane] = K[s][b];
+        if (valid[b]) kv[s * NY + rowv[b]] = K[s][b];
 
     // ---- backward stage recursion (K and Kbar live in LDS, by row) ----
     double yb[NB], pend[NB];
@@ -414,7 +410,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
     for (int s = 0; s < S; ++s)
 #pragma unroll
       for (int b = 0; b < NB; ++b)
-        if (valid[b]) kbv[s * (NV + 1) + b * 64 + lane] = Tab::M[s] * lam[b];
+        if (valid[b]) kbv[s * (NV + 1) + rowv[b]] = Tab::M[s] * lam[b];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       yb[b] = lam[b];
@@ -425,15 +421,15 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
       const int base = s * (s - 1) / 2;
       double kbs[NB], z[NB];
 #pragma unroll
-      for (int b = 0; b < NB; ++b) kbs[b] = valid[b] ? kbv[s * (NV + 1) + b * 64 + lane] : 0.0;
+      for (int b = 0; b < NB; ++b) kbs[b] = valid[b] ? kbv[s * (NV + 1) + rowv[b]] : 0.0;
       int ls = ln;
       asm volatile("" : "+v"(ls));
-      C_::tforward(lu, kbs, rdiag, z, ls);
-      C_::tbackward(lu, z, ls);
+      C_::tforward(lu, rowv, kbs, rdiag, z, ls);
+      C_::tbackward(lu, rowv, z, ls);
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
         if (!valid[b]) z[b] = 0.0;
-        if (valid[b]) rbv[s * (NV + 1) + b * 64 + lane] = z[b];
+        if (valid[b]) rbv[s * (NV + 1) + rowv[b]] = z[b];
         pend[b] += z[b];
       }
       for (int j = 0; j < s; ++j) {
@@ -442,7 +438,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
 #pragma unroll
         for (int b = 0; b < NB; ++b)
           if (valid[b]) {
-            double* kb = &kbv[j * (NV + 1) + b * 64 + lane];
+            double* kb = &kbv[j * (NV + 1) + rowv[b]];
             *kb = fma(c / dt, z[b], *kb);
           }
       }
@@ -455,14 +451,14 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
           if (c == 0.0) continue;
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
-            const double kj = valid[b] ? kv[j * NY + b * 64 + lane] : 0.0;
+            const double kj = valid[b] ? kv[j * NY + rowv[b]] : 0.0;
             Y[b] = (c == 1.0) ? Y[b] + kj : Y[b] + c * kj;
           }
         }
         store_y(Y);
 #pragma unroll
         for (int b = 0; b < NB; ++b)
-          if (valid[b]) uv[b * 64 + lane] = pend[b];
+          if (valid[b]) uv[rowv[b]] = pend[b];
         double Yb[NB];
         jt_pass(Yb);
 #pragma unroll
@@ -476,7 +472,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
 #pragma unroll
           for (int b = 0; b < NB; ++b)
             if (valid[b]) {
-              double* kb = &kbv[j * (NV + 1) + b * 64 + lane];
+              double* kb = &kbv[j * (NV + 1) + rowv[b]];
               *kb = fma(c, Yb[b], *kb);
             }
         }
@@ -492,7 +488,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
   // ---- outputs ----
 #pragma unroll
   for (int b = 0; b < NB; ++b)
-    if (valid[b]) g.lambda[cell + a.stride * (long)T_::VARPHYS[b * 64 + lane]] = lam[b];
+    if (valid[b]) g.lambda[cell + a.stride * (long)T_::VARPHYS[rowv[b]]] = lam[b];
   if (lane < NF) g.lambda[cell + a.stride * (long)T_::FIXPHYS[lane]] = fbar[lane];
 #pragma unroll
   for (int p = 0; p < NGP; ++p)

@@ -105,12 +105,14 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
   double* scr = ylds + W::NY;
   double* facc = scr + W::NSCR;
 
-  int rb[NB];
+  // lane slot (b, lane) holds row ROWOF[b * 64 + lane] (255: none)
+  int rb[NB], rowv[NB];
   bool valid[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    const int row = b * 64 + lane;
-    valid[b] = row < NV;
+    const int row = T_::ROWOF[b * 64 + lane];
+    valid[b] = row != 255;
+    rowv[b] = valid[b] ? row : 0;
     rb[b] = valid[b] ? (int)T_::R[row] : 0;
   }
 
@@ -118,8 +120,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
   double x[NB], atol[NB], rtol[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    const int row = b * 64 + lane;
-    const int ph = valid[b] ? (int)T_::VARPHYS[row] : 0;
+    const int ph = valid[b] ? (int)T_::VARPHYS[rowv[b]] : 0;
     x[b] = valid[b] ? a.conc[cell + a.stride * ph] : 0.0;
     atol[b] = valid[b] ? a.atol[ph] : 1.0;
     rtol[b] = valid[b] ? a.rtol[ph] : 0.0;
@@ -188,7 +189,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
       if (row != 255) facc[row] = facc[row] + acc;
     }
 #pragma unroll
-    for (int b = 0; b < NB; ++b) f[b] = valid[b] ? facc[b * 64 + lane] : 0.0;
+    for (int b = 0; b < NB; ++b) f[b] = valid[b] ? facc[rowv[b]] : 0.0;
   };
 
   // W = I/(g h) - J(y) into the LU storage
@@ -225,7 +226,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
     for (int i = lane; i <= W::SPAN; i += 64) lu[i] = 0.0;
 #pragma unroll
     for (int b = 0; b < NB; ++b)
-      if (valid[b]) lu[rb[b] + b * 64 + lane] = inv_g_dt;
+      if (valid[b]) lu[rb[b] + rowv[b]] = inv_g_dt;
 #pragma unroll
     for (int p = 0; p < C_::NWP; ++p) {
       double acc = 0.0;
@@ -242,7 +243,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
   auto store_y = [&](const double (&y)[NB]) {
 #pragma unroll
     for (int b = 0; b < NB; ++b)
-      if (valid[b]) ylds[b * 64 + lane] = y[b];
+      if (valid[b]) ylds[rowv[b]] = y[b];
   };
 
   // ---------------- integrate `steps` times over dt ----------------
@@ -281,7 +282,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
       assemble_w(inv_g_dt);
       C_::factor(lu, rb, ln);
 #pragma unroll
-      for (int b = 0; b < NB; ++b) rdiag[b] = valid[b] ? 1.0 / lu[rb[b] + b * 64 + lane] : 0.0;
+      for (int b = 0; b < NB; ++b) rdiag[b] = valid[b] ? 1.0 / lu[rb[b] + rowv[b]] : 0.0;
       // Stage loop kept rolled so that one copy of the F and solve code serves
       // every stage.  Coefficients are read by the (uniform) stage index; a
       // zero coefficient skips its term and a unit one adds K bare, exactly
@@ -356,7 +357,10 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
       }
       double err_sq = 0.0;
       if constexpr (MODE != 2)
-        for (int i = 0; i < NV; ++i) err_sq += readlane_d(bank<NB>(term, i), i & 63);
+        for (int i = 0; i < NV; ++i) {
+          const int sl = T_::SLOT[i];  // rows summed in logical order
+          err_sq += readlane_d(bank<NB>(term, sl), sl & 63);
+        }
       double err = sqrt(err_sq / W::N);
       err = fmax(err, 1.0e-10);
       const double factor_ = step_factor<Tab>(err);
@@ -369,7 +373,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
           double* ck = adj->ckpt + (cell * (long)adj->maxsteps + acc_steps) * NV;
 #pragma unroll
           for (int b = 0; b < NB; ++b)
-            if (valid[b]) ck[b * 64 + lane] = x[b];
+            if (valid[b]) ck[rowv[b]] = x[b];
           if (lane == 0) adj->hs[cell * (long)adj->maxsteps + acc_steps] = dt;
         }
 #pragma unroll
@@ -404,7 +408,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
   bool fin = true;
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    if (valid[b]) a.conc[cell + a.stride * (long)T_::VARPHYS[b * 64 + lane]] = x[b];
+    if (valid[b]) a.conc[cell + a.stride * (long)T_::VARPHYS[rowv[b]]] = x[b];
     fin = fin && (!valid[b] || isfinite(x[b]));
   }
   const bool all_fin = __ballot(!fin) == 0ull;
@@ -552,6 +556,8 @@ int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
   template <>                                                                                               \
   struct WaveTables<gen::NAME##_wave::Wave> {                                                               \
     static constexpr const int16_t* R = gen::NAME##_wave::R;                                               \
+    static constexpr const uint8_t* SLOT = gen::NAME##_wave::SLOT;                                          \
+    static constexpr const uint8_t* ROWOF = gen::NAME##_wave::ROWOF;                                        \
     static constexpr const uint8_t* LCNT = gen::NAME##_wave::LCNT;                                          \
     static constexpr const uint16_t* LOFS = gen::NAME##_wave::LOFS;                                         \
     static constexpr const uint8_t* LROW = gen::NAME##_wave::LROW;                                          \

@@ -76,8 +76,34 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
         ucnt.append(len(cols))
         ucol += cols
         uofs.append(len(ucol))
-    # solve masks per column k and bank: rows below (forward) / above (backward)
+    # Row -> (bank, lane) slots.  Lanes are rows; a pivot's update costs one
+    # masked block per bank its rows occupy, so the 64 rows that take part in
+    # the most updates (the trailing, nearly dense block of the LU) share bank
+    # 0 and the rest go to bank 1.  Within a bank rows keep ascending order.
     nb = (nv + 63) // 64
+    part = [0] * nv
+    for k in range(nv):
+        for i in lrow[lofs[k]:lofs[k + 1]]:
+            part[i] += ucnt[k] + 1
+    for (i, k) in U:
+        if i < k:
+            part[i] += 1
+    if nb == 1:
+        bank0 = list(range(nv))
+        bank1 = []
+    else:
+        ranked = sorted(range(nv), key=lambda r: (-part[r], r))
+        bank0 = sorted(ranked[:64])
+        bank1 = sorted(ranked[64:])
+    slot = [0] * nv
+    for lane, r in enumerate(bank0):
+        slot[r] = lane
+    for lane, r in enumerate(bank1):
+        slot[r] = 64 + lane
+    rowof = [255] * (nb * 64)
+    for r in range(nv):
+        rowof[slot[r]] = r
+    # solve masks per column k and bank: rows below (forward) / above (backward)
     fmask = [[0] * nb for _ in range(nv)]
     bmask = [[0] * nb for _ in range(nv)]
     for (i, k) in L:
@@ -216,7 +242,7 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
         "name": p["name"], "n": n, "nv": nv, "nf": nfix, "ng": ng, "nph": p["num_runtime_parameters"],
         "nb": nb, "span": span, "ny": nv + nfix + 1, "npart": npart, "one": ONE,
         "var_phys": vm.var_phys, "fixed_phys": vm.fixed_phys, "third_body": p["third_body"],
-        "R": R, "lcnt": lcnt, "lofs": lofs, "lrow": lrow, "ucnt": ucnt, "uofs": uofs, "ucol": ucol,
+        "slot": slot, "rowof": rowof, "R": R, "lcnt": lcnt, "lofs": lofs, "lrow": lrow, "ucnt": ucnt, "uofs": uofs, "ucol": ucol,
         "fmask": fmask, "bmask": bmask, "law": law, "photo": photo, "params": params, "gfac": gfac,
         "gnpow": gnpow, "gpow_s": gpow_s, "gpow_p": gpow_p, "poff": poff, "coefs": coefs,
         "fofs": fofs, "fg": fg, "fc": fc, "waddr": waddr, "wdiag": wdiag, "wofs": wofs, "wpi": wpi, "wci": wci,
@@ -245,6 +271,8 @@ def emit(t: dict) -> str:
         f"namespace mkpp {{ namespace gen {{ namespace {name}_wave {{",
         "",
         _arr("int16_t", "R", t["R"]),
+        _arr("uint8_t", "SLOT", t["slot"]),
+        _arr("uint8_t", "ROWOF", t["rowof"]),
         _arr("uint8_t", "LCNT", t["lcnt"]),
         _arr("uint16_t", "LOFS", t["lofs"]),
         _arr("uint8_t", "LROW", t["lrow"]),
@@ -447,6 +475,14 @@ def emit_code(t: dict, ct: dict) -> str:
             Lset.add((i, k))
         for j in t["ucol"][t["uofs"][k]:t["uofs"][k + 1]]:
             Uset.add((k, j))
+    slot = t["slot"]
+
+    def masks_of(rows):
+        m = [0] * nb
+        for i in rows:
+            m[slot[i] // 64] |= 1 << (slot[i] % 64)
+        return m
+
     out = []
     w = out.append
     w(f"// Generated by mkpp_amd.gen.emit_wave: straight-line LU and solves for {name}.")
@@ -481,11 +517,9 @@ def emit_code(t: dict, ct: dict) -> str:
         w(f"      const double d = lu[{R[k] + k}];")
         for j in cols:
             w(f"      const double u{j} = lu[{R[k] + j}];")
+        fm = masks_of(rows)
         for b in range(nb):
-            m = 0
-            for i in rows:
-                if i // 64 == b:
-                    m |= 1 << (i % 64)
+            m = fm[b]
             if not m:
                 continue
             w(f"      if ({_lane_test(m)}) {{")
@@ -503,13 +537,10 @@ def emit_code(t: dict, ct: dict) -> str:
     w("  // is reached; rows of L(:, k) subtract L(i,k) x_k (ascending k per row).")
     w("  static __device__ __forceinline__ void forward(const double* __restrict__ lu, const int (&rb)[" + str(nb) + "], double (&wv)[" + str(nb) + "], int lane) {")
     for k in range(nv):
-        masks = [0] * nb
-        for (i, kk) in Lset:
-            if kk == k:
-                masks[i // 64] |= 1 << (i % 64)
+        masks = masks_of([i for (i, kk) in Lset if kk == k])
         if not any(masks):
             continue
-        w(f"    {{ const double xk = readlane_d(wv[{k // 64}], {k % 64});")
+        w(f"    {{ const double xk = readlane_d(wv[{slot[k] // 64}], {slot[k] % 64});")
         for b in range(nb):
             if masks[b]:
                 w(f"      if ({_lane_test(masks[b])}) wv[{b}] = fma(-lu[rb[{b}] + {k}], xk, wv[{b}]);")
@@ -520,12 +551,10 @@ def emit_code(t: dict, ct: dict) -> str:
     w("  // of U(:, k) above the diagonal subtract U(i,k) x_k.")
     w("  static __device__ __forceinline__ void backward(const double* __restrict__ lu, const int (&rb)[" + str(nb) + "], double (&wv)[" + str(nb) + "], const double (&rdiag)[" + str(nb) + "], double (&x)[" + str(nb) + "], int lane) {")
     for k in reversed(range(nv)):
-        masks = [0] * nb
-        for (i, kk) in Uset:
-            if kk == k and i < k:
-                masks[i // 64] |= 1 << (i % 64)
-        w(f"    {{ const double xk = readlane_d(wv[{k // 64}] * rdiag[{k // 64}], {k % 64});")
-        w(f"      if (lane == {k % 64}) x[{k // 64}] = xk;")
+        masks = masks_of([i for (i, kk) in Uset if kk == k and i < k])
+        bk, lk = slot[k] // 64, slot[k] % 64
+        w(f"    {{ const double xk = readlane_d(wv[{bk}] * rdiag[{bk}], {lk});")
+        w(f"      if (lane == {lk}) x[{bk}] = xk;")
         for b in range(nb):
             if masks[b]:
                 w(f"      if ({_lane_test(masks[b])}) wv[{b}] = fma(-lu[rb[{b}] + {k}], xk, wv[{b}]);")
@@ -534,36 +563,31 @@ def emit_code(t: dict, ct: dict) -> str:
     w("")
     # ---- transposed solves (adjoint): W^T = U^T L^T ----
     w("  // U^T z = b, column oriented over U's rows: z_k = b_k / U(k,k) once column k")
-    w("  // is reached, then every j > k in row k of U subtracts U(k,j) z_k.  Lane j")
-    w("  // reads U(k,j) at R[k] + j, a contiguous run across the lanes.")
-    w("  static __device__ __forceinline__ void tforward(const double* __restrict__ lu, double (&b)[" + str(nb) + "], const double (&rdiag)[" + str(nb) + "], double (&z)[" + str(nb) + "], int lane) {")
+    w("  // is reached, then every j > k in row k of U subtracts U(k,j) z_k.  The lane")
+    w("  // holding row j (rv) reads U(k,j) at R[k] + j.")
+    w("  static __device__ __forceinline__ void tforward(const double* __restrict__ lu, const int (&rv)[" + str(nb) + "], double (&b)[" + str(nb) + "], const double (&rdiag)[" + str(nb) + "], double (&z)[" + str(nb) + "], int lane) {")
     for k in range(nv):
-        masks = [0] * nb
-        for (kk, j) in Uset:
-            if kk == k and j > k:
-                masks[j // 64] |= 1 << (j % 64)
-        w(f"    {{ const double zk = readlane_d(b[{k // 64}] * rdiag[{k // 64}], {k % 64});")
-        w(f"      if (lane == {k % 64}) z[{k // 64}] = zk;")
+        masks = masks_of([j for (kk, j) in Uset if kk == k and j > k])
+        bk, lk = slot[k] // 64, slot[k] % 64
+        w(f"    {{ const double zk = readlane_d(b[{bk}] * rdiag[{bk}], {lk});")
+        w(f"      if (lane == {lk}) z[{bk}] = zk;")
         for bb in range(nb):
             if masks[bb]:
-                w(f"      if ({_lane_test(masks[bb])}) b[{bb}] = fma(-lu[{R[k] + bb * 64} + lane], zk, b[{bb}]);")
+                w(f"      if ({_lane_test(masks[bb])}) b[{bb}] = fma(-lu[{R[k]} + rv[{bb}]], zk, b[{bb}]);")
         w("    }")
     w("  }")
     w("")
     w("  // L^T x = z in place (unit diagonal): x_k = z_k when column k is reached,")
     w("  // descending; every j < k in row k of L subtracts L(k,j) x_k.")
-    w("  static __device__ __forceinline__ void tbackward(const double* __restrict__ lu, double (&z)[" + str(nb) + "], int lane) {")
+    w("  static __device__ __forceinline__ void tbackward(const double* __restrict__ lu, const int (&rv)[" + str(nb) + "], double (&z)[" + str(nb) + "], int lane) {")
     for k in reversed(range(nv)):
-        masks = [0] * nb
-        for (kk, j) in Lset:
-            if kk == k:
-                masks[j // 64] |= 1 << (j % 64)
+        masks = masks_of([j for (kk, j) in Lset if kk == k])
         if not any(masks):
             continue
-        w(f"    {{ const double xk = readlane_d(z[{k // 64}], {k % 64});")
+        w(f"    {{ const double xk = readlane_d(z[{slot[k] // 64}], {slot[k] % 64});")
         for bb in range(nb):
             if masks[bb]:
-                w(f"      if ({_lane_test(masks[bb])}) z[{bb}] = fma(-lu[{R[k] + bb * 64} + lane], xk, z[{bb}]);")
+                w(f"      if ({_lane_test(masks[bb])}) z[{bb}] = fma(-lu[{R[k]} + rv[{bb}]], xk, z[{bb}]);")
         w("    }")
     w("  }")
     w("};")

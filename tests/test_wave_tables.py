@@ -41,6 +41,9 @@ def test_indices_in_bounds(case):
     # row displacement: no two structural non-zeros share an address
     assert len(t["waddr"]) == len(set(t["waddr"]))
     assert ng <= 255 and nv <= 128  # uint8 group / row indices
+    # row -> (bank, lane) slot map is a bijection onto the used slots
+    assert len(set(t["slot"])) == nv and all(t["rowof"][t["slot"][r]] == r for r in range(nv))
+    assert sum(1 for r in t["rowof"] if r != 255) == nv and max(t["slot"]) < 64 * t["nb"]
     # adjoint tables: group -> variable rows, species slot -> partial slots
     assert t["gnvofs"][-1] == len(t["gnvr"]) and max(t["gnvr"]) < nv and max(t["gnvc"]) < len(t["coefs"])
     assert len(t["spofs"]) == nv + t["nf"] + 1 and t["spofs"][-1] == len(t["spi"]) == npart
