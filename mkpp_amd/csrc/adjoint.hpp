@@ -27,6 +27,8 @@
 // Fixed species take part in the fluxes but not in the state: their adjoint
 // accumulates over all steps.  Rate constants are per flux group.
 #pragma once
+#include <type_traits>
+
 #include "wave_engine.hpp"
 
 namespace mkpp {
@@ -35,7 +37,8 @@ namespace mkpp {
 // stage, the u vector of a J^T pass and the fixed-species adjoint.
 template <class W, class Tab>
 constexpr int adjoint_lds_doubles() {
-  return W::LDS_DOUBLES + 2 * Tab::S * (W::NV + 1) + Tab::S * W::NY + (W::NV + 1) + (W::NF + 1) + W::NY;
+  return W::LDS_DOUBLES + 2 * Tab::S * (W::NV + 1) + Tab::S * W::NY + (W::NV + 1) + (W::NF + 1) + W::NY +
+         (W::NG + 1);
 }
 
 template <class W, class Tab>
@@ -54,6 +57,8 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
   double* uv = kbv + S * (NV + 1);   // [NV+1]    u of a J^T pass by row
   double* fbar = uv + (NV + 1);      // [NF]      adjoint of the fixed species
   double* sacc = fbar + (NF + 1);    // [NY]      scatter sums per species slot
+  double* wj = sacc + NY;            // [NG+1]    nu_g . u of a J^T pass
+  double* wh = lu;                   // [S][NG+1] nu_g . rbar_s (second-derivative pass, after the LU is done)
 
   // lane slot (b, lane) holds row ROWOF[b * 64 + lane] (255: none)
   int rb[NB], rowv[NB];
@@ -97,6 +102,9 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
   for (int b = 0; b < NB; ++b) lam[b] = valid[b] ? g.lambda[cell + a.stride * (long)T_::VARPHYS[rowv[b]]] : 0.0;
   if (lane < NF) fbar[lane] = g.lambda[cell + a.stride * (long)T_::FIXPHYS[lane]];
   for (int i = lane; i < S * NY; i += 64) kv[i] = 0.0;
+  // slot NV of the by-row vectors stays 0.0: the padding terms of the gather read it
+  if (lane < S) rbv[lane * (NV + 1) + NV] = 0.0;
+  if (lane == 0) uv[NV] = 0.0;
 
   auto store_y = [&](const double (&y)[NB]) {
 #pragma unroll
@@ -191,11 +199,31 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
     }
   };
 
-  // nu_g . u for every group of this lane, u by row in LDS
-  auto nu_dot = [&](int gi, const double* u) {
-    double w = 0.0;
-    for (int t = T_::GNVOFS[gi]; t < T_::GNVOFS[gi + 1]; ++t) w = fma(T_::COEF[T_::GNVC[t]], u[T_::GNVR[t]], w);
-    return w;
+  // out[s * (NG + 1) + g] = nu_g . u_s for every group g and the NVEC vectors
+  // u_s = u + s * (NV + 1) (by row).  Fixed-trip chunks (CG tables): one chunk of
+  // a group's (row, coefficient) terms per lane and pass.
+  auto group_dots = [&](auto nvec_c, const double* u, double* out) {
+    constexpr int NVEC = decltype(nvec_c)::value;
+    for (int i = lane; i < NVEC * (NG + 1); i += 64) out[i] = 0.0;
+#pragma unroll
+    for (int p = 0; p < C_::NGAT; ++p) {
+      double acc[NVEC];
+#pragma unroll
+      for (int s = 0; s < NVEC; ++s) acc[s] = 0.0;
+#pragma unroll
+      for (int q = 0; q < C_::TG; ++q) {
+        const uint32_t tm = T_::CG_TERM[(p * C_::TG + q) * 64 + lane];
+        const double cf = T_::COEF[tm >> 8];
+        const int r = tm & 255u;
+#pragma unroll
+        for (int s = 0; s < NVEC; ++s) acc[s] = fma(cf, u[s * (NV + 1) + r], acc[s]);
+      }
+      const int gg = T_::CG_GRP[p * 64 + lane];
+      if (gg != 255) {
+#pragma unroll
+        for (int s = 0; s < NVEC; ++s) out[s * (NG + 1) + gg] += acc[s];
+      }
+    }
   };
 
   // scr[partial slot] summed per species slot: rows into out[], fixed species
@@ -235,11 +263,12 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
   };
 
   auto jt_pass = [&](double (&out)[NB]) {
+    group_dots(std::integral_constant<int, 1>{}, uv, wj);
 #pragma unroll 1
     for (int p = 0; p < NGP; ++p) {
       const int gi = p * 64 + lane;
       if (gi < NG) {
-        const double w = nu_dot(gi, uv);
+        const double w = wj[gi];
         double pw[3], pm1[3], pm2[3];
         monomial(gi, pw, pm1, pm2);
         kbar_add(p, w * ((pw[0] * pw[1]) * pw[2]));
@@ -261,6 +290,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
 
   // second-derivative terms at y_n: sum_s d/dy [rbar_s^T J K_s] and d/dk
   auto hess_pass = [&](double (&out)[NB]) {
+    group_dots(std::integral_constant<int, S>{}, rbv, wh);
 #pragma unroll 1
     for (int p = 0; p < NGP; ++p) {
       const int gi = p * 64 + lane;
@@ -294,16 +324,9 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
             d2[e][f] = h;
           }
         }
-        // nu_g . rbar_s for every stage in one pass over the group's rows
         double ws[S];
 #pragma unroll
-        for (int s = 0; s < S; ++s) ws[s] = 0.0;
-        for (int t = T_::GNVOFS[gi]; t < T_::GNVOFS[gi + 1]; ++t) {
-          const double cf = T_::COEF[T_::GNVC[t]];
-          const int r = T_::GNVR[t];
-#pragma unroll
-          for (int s = 0; s < S; ++s) ws[s] = fma(cf, rbv[s * (NV + 1) + r], ws[s]);
-        }
+        for (int s = 0; s < S; ++s) ws[s] = wh[s * (NG + 1) + gi];
         double val[3] = {0.0, 0.0, 0.0};
         double kb = 0.0;
 #pragma unroll

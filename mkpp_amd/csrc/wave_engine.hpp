@@ -603,6 +603,8 @@ int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
     static constexpr const uint16_t* SPI = gen::NAME##_wave::SPI;                                           \
     static constexpr const uint16_t* CS_TERM = gen::NAME##_wave::CS_TERM;                                   \
     static constexpr const uint8_t* CS_ROW = gen::NAME##_wave::CS_ROW;                                      \
+    static constexpr const uint16_t* CG_TERM = gen::NAME##_wave::CG_TERM;                                   \
+    static constexpr const uint8_t* CG_GRP = gen::NAME##_wave::CG_GRP;                                      \
     using Code = gen::NAME##_wave::Code;                                                                    \
   };                                                                                                        \
   }

@@ -460,8 +460,30 @@ def build_code_tables(t: dict, tf: int = 4, tw: int = 2) -> dict:
                 sterm.append(ps[lane][1][q] if lane < len(ps) and q < len(ps[lane][1]) else ZW)
         for lane in range(64):
             srow.append(ps[lane][0] if lane < len(ps) else 255)
+    # adjoint gather nu_g . u: chunks of (row, coefficient) terms per group;
+    # padding reads row NV (an always-zero slot) with coefficient 0
+    tg = 4
+    gitems = []
+    for g in range(t["ng"]):
+        terms = [(t["gnvr"][k], t["gnvc"][k]) for k in range(t["gnvofs"][g], t["gnvofs"][g + 1])]
+        if terms:
+            gitems.append((g, terms))
+    gpasses = _pack_chunks(gitems, tg)
+    gterm, ggrp = [], []
+    for ps in gpasses:
+        for q in range(tg):
+            for lane in range(64):
+                if lane < len(ps) and q < len(ps[lane][1]):
+                    r, ci = ps[lane][1][q]
+                    gterm.append(r | (ci << 8))
+                else:
+                    gterm.append(t["nv"] | (zero_c << 8))
+        for lane in range(64):
+            ggrp.append(ps[lane][0] if lane < len(ps) else 255)
+    assert t["nv"] < 256 and len(coefs) <= 256
     return {"tf": tf, "tw": tw, "nfp": len(fpasses), "nwp": len(wpasses), "fterm": fterm, "frow": frow,
-            "wterm": wterm, "wkey": waddr, "ts": ts, "nsp": len(spasses), "sterm": sterm, "srow": srow}
+            "wterm": wterm, "wkey": waddr, "ts": ts, "nsp": len(spasses), "sterm": sterm, "srow": srow,
+            "tg": tg, "ngat": len(gpasses), "gterm": gterm, "ggrp": ggrp}
 
 
 def emit_code(t: dict, ct: dict) -> str:
@@ -498,10 +520,13 @@ def emit_code(t: dict, ct: dict) -> str:
     w(_arr("uint16_t", "CW_ADDR", ct["wkey"]))
     w(_arr("uint16_t", "CS_TERM", ct["sterm"]))
     w(_arr("uint8_t", "CS_ROW", ct["srow"]))
+    w(_arr("uint16_t", "CG_TERM", ct["gterm"]))
+    w(_arr("uint8_t", "CG_GRP", ct["ggrp"]))
     w("")
     w("struct Code {")
     w(f"  static constexpr int TF = {ct['tf']}, NFP = {ct['nfp']}, TW = {ct['tw']}, NWP = {ct['nwp']};")
     w(f"  static constexpr int TS = {ct['ts']}, NSP = {ct['nsp']};  // adjoint scatter chunks")
+    w(f"  static constexpr int TG = {ct['tg']}, NGAT = {ct['ngat']};  // adjoint gather chunks")
     w("")
     # ---- factorization ----
     w("  // Right-looking factorization: pivot k divides column k of every row in")
