@@ -558,6 +558,15 @@ def emit_code(t: dict, ct: dict) -> str:
     w("  }")
     w("")
     # ---- forward and backward substitution ----
+    # The solves load their factor entry in every lane, scale it by 1 in the
+    # column's rows and by 0 elsewhere (a select of constants, not a branch)
+    # and update every lane: fma(-0, x, w) = w for finite x, and the loads and
+    # scalings, which depend on nothing, run ahead of the substitution chain.  Every such address, R[row] + k or
+    # R[k] + row with row, k < NV, lies inside the wavefront's LDS block.
+    lds_doubles = (t["span"] + 1) + t["ny"] + (max(t["ng"], t["npart"]) + 1) + (nv + 1)
+    assert min(R) >= 0 and max(R) + nv - 1 < lds_doubles
+    w("  // Solves: every lane loads its entry (in bounds for any row and column),")
+    w("  // zeroed outside the column's rows by a factor off the substitution chain.")
     w("  // Forward substitution, column oriented: x_k = w_k is final when column k")
     w("  // is reached; rows of L(:, k) subtract L(i,k) x_k (ascending k per row).")
     w("  static __device__ __forceinline__ void forward(const double* __restrict__ lu, const int (&rb)[" + str(nb) + "], double (&wv)[" + str(nb) + "], int lane) {")
@@ -568,7 +577,8 @@ def emit_code(t: dict, ct: dict) -> str:
         w(f"    {{ const double xk = readlane_d(wv[{slot[k] // 64}], {slot[k] % 64});")
         for b in range(nb):
             if masks[b]:
-                w(f"      if ({_lane_test(masks[b])}) wv[{b}] = fma(-lu[rb[{b}] + {k}], xk, wv[{b}]);")
+                w(f"      {{ const double l = lu[rb[{b}] + {k}] * ({_lane_test(masks[b])} ? 1.0 : 0.0);")
+                w(f"        wv[{b}] = fma(-l, xk, wv[{b}]); }}")
         w("    }")
     w("  }")
     w("")
@@ -579,10 +589,11 @@ def emit_code(t: dict, ct: dict) -> str:
         masks = masks_of([i for (i, kk) in Uset if kk == k and i < k])
         bk, lk = slot[k] // 64, slot[k] % 64
         w(f"    {{ const double xk = readlane_d(wv[{bk}] * rdiag[{bk}], {lk});")
-        w(f"      if (lane == {lk}) x[{bk}] = xk;")
+        w(f"      x[{bk}] = (lane == {lk}) ? xk : x[{bk}];")
         for b in range(nb):
             if masks[b]:
-                w(f"      if ({_lane_test(masks[b])}) wv[{b}] = fma(-lu[rb[{b}] + {k}], xk, wv[{b}]);")
+                w(f"      {{ const double u = lu[rb[{b}] + {k}] * ({_lane_test(masks[b])} ? 1.0 : 0.0);")
+                w(f"        wv[{b}] = fma(-u, xk, wv[{b}]); }}")
         w("    }")
     w("  }")
     w("")
@@ -595,10 +606,11 @@ def emit_code(t: dict, ct: dict) -> str:
         masks = masks_of([j for (kk, j) in Uset if kk == k and j > k])
         bk, lk = slot[k] // 64, slot[k] % 64
         w(f"    {{ const double zk = readlane_d(b[{bk}] * rdiag[{bk}], {lk});")
-        w(f"      if (lane == {lk}) z[{bk}] = zk;")
+        w(f"      z[{bk}] = (lane == {lk}) ? zk : z[{bk}];")
         for bb in range(nb):
             if masks[bb]:
-                w(f"      if ({_lane_test(masks[bb])}) b[{bb}] = fma(-lu[{R[k]} + rv[{bb}]], zk, b[{bb}]);")
+                w(f"      {{ const double u = lu[{R[k]} + rv[{bb}]] * ({_lane_test(masks[bb])} ? 1.0 : 0.0);")
+                w(f"        b[{bb}] = fma(-u, zk, b[{bb}]); }}")
         w("    }")
     w("  }")
     w("")
@@ -612,7 +624,8 @@ def emit_code(t: dict, ct: dict) -> str:
         w(f"    {{ const double xk = readlane_d(z[{slot[k] // 64}], {slot[k] % 64});")
         for bb in range(nb):
             if masks[bb]:
-                w(f"      if ({_lane_test(masks[bb])}) z[{bb}] = fma(-lu[{R[k]} + rv[{bb}]], xk, z[{bb}]);")
+                w(f"      {{ const double l = lu[{R[k]} + rv[{bb}]] * ({_lane_test(masks[bb])} ? 1.0 : 0.0);")
+                w(f"        z[{bb}] = fma(-l, xk, z[{bb}]); }}")
         w("    }")
     w("  }")
     w("};")
