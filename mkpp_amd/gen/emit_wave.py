@@ -88,13 +88,16 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
     for (i, k) in U:
         if i < k:
             part[i] += 1
+    # the trailing block factorized in registers (rows >= k0) must sit in bank 0
+    k0 = max(0, nv - REGISTER_BLOCK)
     if nb == 1:
         bank0 = list(range(nv))
         bank1 = []
     else:
-        ranked = sorted(range(nv), key=lambda r: (-part[r], r))
-        bank0 = sorted(ranked[:64])
-        bank1 = sorted(ranked[64:])
+        forced = list(range(k0, nv))
+        ranked = [r for r in sorted(range(nv), key=lambda r: (-part[r], r)) if r < k0]
+        bank0 = sorted(forced + ranked[:64 - len(forced)])
+        bank1 = sorted(ranked[64 - len(forced):])
     slot = [0] * nv
     for lane, r in enumerate(bank0):
         slot[r] = lane
@@ -242,7 +245,7 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
         "name": p["name"], "n": n, "nv": nv, "nf": nfix, "ng": ng, "nph": p["num_runtime_parameters"],
         "nb": nb, "span": span, "ny": nv + nfix + 1, "npart": npart, "one": ONE,
         "var_phys": vm.var_phys, "fixed_phys": vm.fixed_phys, "third_body": p["third_body"],
-        "slot": slot, "rowof": rowof, "R": R, "lcnt": lcnt, "lofs": lofs, "lrow": lrow, "ucnt": ucnt, "uofs": uofs, "ucol": ucol,
+        "slot": slot, "rowof": rowof, "k0": k0, "R": R, "lcnt": lcnt, "lofs": lofs, "lrow": lrow, "ucnt": ucnt, "uofs": uofs, "ucol": ucol,
         "fmask": fmask, "bmask": bmask, "law": law, "photo": photo, "params": params, "gfac": gfac,
         "gnpow": gnpow, "gpow_s": gpow_s, "gpow_p": gpow_p, "poff": poff, "coefs": coefs,
         "fofs": fofs, "fg": fg, "fc": fc, "waddr": waddr, "wdiag": wdiag, "wofs": wofs, "wpi": wpi, "wci": wci,
@@ -362,6 +365,11 @@ def write(mech: Mechanism, p: dict, outdir: str) -> list:
 # fixed-trip-count term tables for F and W (the compiler can batch-issue every
 # table load and LDS access because no loop bound depends on data).
 # ---------------------------------------------------------------------------
+# Trailing rows of the LU factorized in registers (lanes = rows, one register per
+# column); 35 columns of doubles are 70 VGPRs.
+REGISTER_BLOCK = 35
+
+
 def _lane_test(mask: int, var: str = "lane") -> str:
     """C condition selecting the lanes of a 64-bit mask."""
     if mask == 0:
@@ -533,15 +541,21 @@ def emit_code(t: dict, ct: dict) -> str:
     w("  // L(k) by U(k,k) and subtracts l * U(k, j) for j in U(k), each product in")
     w("  // ascending-k order per entry (the reference's per-entry sequence).")
     w("  static __device__ __forceinline__ void factor(double* __restrict__ lu, const int (&rb)[" + str(nb) + "], int lane) {")
-    for k in range(nv):
+    w("    // pivot rows are read through an opaque base: with a constant address the")
+    w("    // compiler hoists every one out of the step loop and spills them")
+    w("    int zero = 0;")
+    w("    asm volatile(\"\" : \"+v\"(zero));")
+    w("    const double* __restrict__ pv = lu + zero;")
+    k0 = t["k0"]
+    for k in range(k0):
         rows = t["lrow"][t["lofs"][k]:t["lofs"][k + 1]]
         if not rows:
             continue
         cols = t["ucol"][t["uofs"][k]:t["uofs"][k + 1]]
         w(f"    {{  // pivot {k}: {len(rows)} rows x {len(cols)} columns")
-        w(f"      const double d = lu[{R[k] + k}];")
+        w(f"      const double d = pv[{R[k] + k}];")
         for j in cols:
-            w(f"      const double u{j} = lu[{R[k] + j}];")
+            w(f"      const double u{j} = pv[{R[k] + j}];")
         fm = masks_of(rows)
         for b in range(nb):
             m = fm[b]
@@ -555,6 +569,34 @@ def emit_code(t: dict, ct: dict) -> str:
                 w(f"        r[{j}] = fma(-l, u{j}, r[{j}]);")
             w("      }")
         w("    }")
+    if k0 < nv:
+        # trailing block: rows and columns >= k0, all rows in bank 0
+        assert all(slot[r] < 64 for r in range(k0, nv))
+        pattern = set(Lset) | set(Uset) | {(i, i) for i in range(nv)}
+        w(f"    // Rows and columns {k0}..{nv - 1} in registers: lane = row (bank 0), c<j> =")
+        w("    // the row's entry in column j (0 where the pattern has none).  Pivot rows")
+        w("    // are broadcast with readlane; the multiplier is zeroed outside L(k), so")
+        w("    // every update is an unconditional fma that leaves other lanes unchanged.")
+        for c in range(k0, nv):
+            m = masks_of([r for r in range(k0, nv) if (r, c) in pattern])[0]
+            w(f"    double c{c} = lu[rb[0] + {c}] * ({_lane_test(m)} ? 1.0 : 0.0);")
+        for k in range(k0, nv):
+            rows = t["lrow"][t["lofs"][k]:t["lofs"][k + 1]]
+            if not rows:
+                continue
+            cols = t["ucol"][t["uofs"][k]:t["uofs"][k + 1]]
+            m = masks_of(rows)[0]
+            w(f"    {{  // pivot {k}: {len(rows)} rows x {len(cols)} columns")
+            w(f"      const double d = readlane_d(c{k}, {slot[k]});")
+            w(f"      const bool in_l = {_lane_test(m)};")
+            w(f"      const double l = in_l ? c{k} / d : 0.0;")
+            w(f"      c{k} = in_l ? l : c{k};")
+            for j in cols:
+                w(f"      c{j} = fma(-l, readlane_d(c{j}, {slot[k]}), c{j});")
+            w("    }")
+        for c in range(k0, nv):
+            m = masks_of([r for r in range(k0, nv) if (r, c) in pattern])[0]
+            w(f"    if ({_lane_test(m)}) lu[rb[0] + {c}] = c{c};")
     w("  }")
     w("")
     # ---- forward and backward substitution ----
