@@ -37,7 +37,7 @@ namespace mkpp {
 // stage, the u vector of a J^T pass and the fixed-species adjoint.
 template <class W, class Tab>
 constexpr int adjoint_lds_doubles() {
-  return W::LDS_DOUBLES + 2 * Tab::S * (W::NV + 1) + Tab::S * W::NY + (W::NV + 1) + (W::NF + 1) + W::NY +
+  return W::LDS_DOUBLES + (2 * Tab::S + 1) * (W::NV + 1) + Tab::S * W::NY + (W::NV + 1) + (W::NF + 1) + W::NY +
          (W::NG + 1);
 }
 
@@ -51,14 +51,14 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
   double* ylds = lu + (W::SPAN + 1);
   double* scr = ylds + NY;
   double* facc = scr + W::NSCR;
-  double* rbv = facc + (NV + 1);     // [S][NV+1] rbar_s by row
-  double* kv = rbv + S * (NV + 1);   // [S][NY]   K_s by y slot (fixed species and ONE: 0)
+  double* rbv = facc + (NV + 1);     // [S+1][NV+1] rbar_s by row; row S: Fbar of stage 0
+  double* kv = rbv + (S + 1) * (NV + 1);  // [S][NY] K_s by y slot (fixed species and ONE: 0)
   double* kbv = kv + S * NY;         // [S][NV+1] Kbar_s by row
   double* uv = kbv + S * (NV + 1);   // [NV+1]    u of a J^T pass by row
   double* fbar = uv + (NV + 1);      // [NF]      adjoint of the fixed species
   double* sacc = fbar + (NF + 1);    // [NY]      scatter sums per species slot
   double* wj = sacc + NY;            // [NG+1]    nu_g . u of a J^T pass
-  double* wh = lu;                   // [S][NG+1] nu_g . rbar_s (second-derivative pass, after the LU is done)
+  double* wh = lu;                   // [S+1][NG+1] nu_g . rbv rows (second-derivative pass, after the LU is done)
 
   // lane slot (b, lane) holds row ROWOF[b * 64 + lane] (255: none)
   int rb[NB], rowv[NB];
@@ -103,7 +103,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
   if (lane < NF) fbar[lane] = g.lambda[cell + a.stride * (long)T_::FIXPHYS[lane]];
   for (int i = lane; i < S * NY; i += 64) kv[i] = 0.0;
   // slot NV of the by-row vectors stays 0.0: the padding terms of the gather read it
-  if (lane < S) rbv[lane * (NV + 1) + NV] = 0.0;
+  if (lane <= S) rbv[lane * (NV + 1) + NV] = 0.0;
   if (lane == 0) uv[NV] = 0.0;
 
   auto store_y = [&](const double (&y)[NB]) {
@@ -264,7 +264,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
 
   auto jt_pass = [&](double (&out)[NB]) {
     group_dots(std::integral_constant<int, 1>{}, uv, wj);
-#pragma unroll 1
+#pragma unroll
     for (int p = 0; p < NGP; ++p) {
       const int gi = p * 64 + lane;
       if (gi < NG) {
@@ -289,8 +289,10 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
   };
 
   // second-derivative terms at y_n: sum_s d/dy [rbar_s^T J K_s] and d/dk
+  // Also carries the J^T pass of stage 0, which is evaluated at y_n too: its
+  // Fbar is row S of rbv, and its terms join the same partial slots.
   auto hess_pass = [&](double (&out)[NB]) {
-    group_dots(std::integral_constant<int, S>{}, rbv, wh);
+    group_dots(std::integral_constant<int, S + 1>{}, rbv, wh);
 #pragma unroll 1
     for (int p = 0; p < NGP; ++p) {
       const int gi = p * 64 + lane;
@@ -327,8 +329,9 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
         double ws[S];
 #pragma unroll
         for (int s = 0; s < S; ++s) ws[s] = wh[s * (NG + 1) + gi];
-        double val[3] = {0.0, 0.0, 0.0};
-        double kb = 0.0;
+        const double w0 = wh[S * (NG + 1) + gi];
+        double val[3] = {w0 * d1[0], w0 * d1[1], w0 * d1[2]};
+        double kb = w0 * ((pw[0] * pw[1]) * pw[2]);
 #pragma unroll
         for (int s = 0; s < S; ++s) {
           const double w = ws[s];
@@ -479,6 +482,13 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
           }
         }
         store_y(Y);
+        if (s == 0) {
+          // stage 0 (Y = y_n): its J^T pass is merged into the second-derivative pass
+#pragma unroll
+          for (int b = 0; b < NB; ++b)
+            if (valid[b]) rbv[S * (NV + 1) + rowv[b]] = pend[b];
+          continue;
+        }
 #pragma unroll
         for (int b = 0; b < NB; ++b)
           if (valid[b]) uv[rowv[b]] = pend[b];
