@@ -528,19 +528,19 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
     if (p * 64 + lane < NG) g.kbar[cell + g.kstride * (long)(p * 64 + lane)] = kbar[p];
 }
 
+// One cell per wavefront at the XCD-aware cell order of the forward kernel.
 template <class W, class Tab>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) wave_adjoint_kernel(DeviceArgs a, AdjArgs g) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) wave_adjoint_kernel(DeviceArgs a, AdjArgs g, long base) {
   extern __shared__ double lds[];
-  const long first = (long)(blockIdx.x % 8u) * (gridDim.x / 8u) + blockIdx.x / 8u;
-  for (long cell = first; cell < a.ncells; cell += gridDim.x) wave_adjoint_cell<W, Tab>(a, g, cell, lds, threadIdx.x);
+  const long cell = wave_cell_of_block(base);
+  if (cell < a.ncells) wave_adjoint_cell<W, Tab>(a, g, cell, lds, threadIdx.x);
 }
 
 template <class W, class Tab, int MODE>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) wave_record_kernel(DeviceArgs a, AdjArgs g) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) wave_record_kernel(DeviceArgs a, AdjArgs g, long base) {
   extern __shared__ double lds[];
-  const long first = (long)(blockIdx.x % 8u) * (gridDim.x / 8u) + blockIdx.x / 8u;
-  for (long cell = first; cell < a.ncells; cell += gridDim.x)
-    wave_integrate_cell<W, Tab, MODE>(a, cell, lds, threadIdx.x, &g);
+  const long cell = wave_cell_of_block(base);
+  if (cell < a.ncells) wave_integrate_cell<W, Tab, MODE>(a, cell, lds, threadIdx.x, &g);
 }
 
 // Rate constants per flux group and cell (the values the laws give; the
@@ -566,21 +566,23 @@ __global__ void __launch_bounds__(64) wave_rate_constants_kernel(DeviceArgs a, d
 template <class W>
 int launch_wave_diff(int mode, const DeviceArgs& a, const AdjArgs& g, hipStream_t s) {
   if (a.ncells <= 0) return 0;
-  const long cells = a.ncells < (1L << 30) ? a.ncells : (1L << 30);
-  const unsigned grid = (unsigned)((cells + 7) / 8 * 8);
   if (mode == 0) {
     wave_rate_constants_kernel<W><<<(unsigned)((a.ncells + 63) / 64), 64, 0, s>>>(a, g.kbar, g.kstride);
     return hipGetLastError() == hipSuccess ? 0 : -2;
   }
   const size_t fwd_lds = sizeof(double) * W::LDS_DOUBLES;
-#define MKPP_DIFF_CASE(TAB)                                                                        \
-  if (mode == 1) wave_record_kernel<W, TAB, 1><<<grid, 64, fwd_lds, s>>>(a, g);                    \
-  else if (mode == 2) wave_record_kernel<W, TAB, 2><<<grid, 64, fwd_lds, s>>>(a, g);               \
-  else wave_adjoint_kernel<W, TAB><<<grid, 64, sizeof(double) * adjoint_lds_doubles<W, TAB>(), s>>>(a, g);
-  switch (a.method) {
-    case RODAS3: MKPP_DIFF_CASE(TabRodas3) break;
-    case ROS4: MKPP_DIFF_CASE(TabRos4) break;
-    default: return -1;
+#define MKPP_DIFF_CASE(TAB)                                                                              \
+  if (mode == 1) wave_record_kernel<W, TAB, 1><<<grid, 64, fwd_lds, s>>>(a, g, base);                    \
+  else if (mode == 2) wave_record_kernel<W, TAB, 2><<<grid, 64, fwd_lds, s>>>(a, g, base);               \
+  else wave_adjoint_kernel<W, TAB><<<grid, 64, sizeof(double) * adjoint_lds_doubles<W, TAB>(), s>>>(a, g, base);
+  if (a.method != RODAS3 && a.method != ROS4) return -1;
+  for (long base = 0; base < a.ncells; base += WAVE_CHUNK_CELLS) {
+    const unsigned grid = wave_chunk_grid(a.ncells, base);
+    if (a.method == RODAS3) {
+      MKPP_DIFF_CASE(TabRodas3)
+    } else {
+      MKPP_DIFF_CASE(TabRos4)
+    }
   }
 #undef MKPP_DIFF_CASE
   return hipGetLastError() == hipSuccess ? 0 : -2;

@@ -117,13 +117,11 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
   }
 
   // ---------------- load ----------------
-  double x[NB], atol[NB], rtol[NB];
+  double x[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int ph = valid[b] ? (int)T_::VARPHYS[rowv[b]] : 0;
     x[b] = valid[b] ? a.conc[cell + a.stride * ph] : 0.0;
-    atol[b] = valid[b] ? a.atol[ph] : 1.0;
-    rtol[b] = valid[b] ? a.rtol[ph] : 0.0;
   }
   if (lane < W::NF) ylds[NV + lane] = a.conc[cell + a.stride * (long)T_::FIXPHYS[lane]];
   if (lane == 0) ylds[W::ONE] = 1.0;
@@ -352,7 +350,10 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
           yerr = first ? tt : yerr + tt;
           first = false;
         }
-        const double sci = atol[b] + rtol[b] * fmax(fabs(x[b]), fabs(ynew));
+        // tolerances are read where used (a few cache lines shared by every
+        // cell) rather than held in registers through the step
+        const int ph = valid[b] ? (int)T_::VARPHYS[rowv[b]] : 0;
+        const double sci = a.atol[ph] + a.rtol[ph] * fmax(fabs(x[b]), fabs(ynew));
         term[b] = valid[b] ? (yerr / sci) * (yerr / sci) : 0.0;
       }
       double err_sq = 0.0;
@@ -421,17 +422,32 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
   }
 }
 
-// One cell per wavefront over a grid-stride loop.
+// XCD-aware cell order: workgroups go round-robin to the 8 XCDs, so block b
+// runs on XCD b % 8.  Giving each XCD a contiguous range of cells lets the 16
+// cells that share a 128-byte line of every species row (conc is
+// species-major) be served by one L2 instead of eight.  gridDim.x is a multiple
+// of 8, which makes the map a bijection on [0, gridDim.x).
+__device__ inline long wave_cell_of_block(long base) {
+  return base + (long)(blockIdx.x % 8u) * (gridDim.x / 8u) + blockIdx.x / 8u;
+}
+
+// One cell per wavefront, no loop: a launch covers cells [base, base + grid).
+// Without an enclosing loop the compiler has nothing to hoist the math
+// library's polynomial constants out of, so they are not kept live (and
+// spilled to scratch, 8.7 KB per wave) across the whole integration.
 template <class W, class Tab>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) wave_integrate_kernel(DeviceArgs a) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) wave_integrate_kernel(DeviceArgs a, long base) {
   extern __shared__ double lds[];
-  // XCD-aware cell order: workgroups go round-robin to the 8 XCDs, so block b
-  // runs on XCD b % 8.  Giving each XCD a contiguous range of cells lets the
-  // 16 cells that share a 128-byte line of every species row (conc is
-  // species-major) be served by one L2 instead of eight.  gridDim.x is a
-  // multiple of 8, which makes the map a bijection on [0, gridDim.x).
-  const long first = (long)(blockIdx.x % 8u) * (gridDim.x / 8u) + blockIdx.x / 8u;
-  for (long cell = first; cell < a.ncells; cell += gridDim.x) wave_integrate_cell<W, Tab>(a, cell, lds, threadIdx.x);
+  const long cell = wave_cell_of_block(base);
+  if (cell < a.ncells) wave_integrate_cell<W, Tab>(a, cell, lds, threadIdx.x);
+}
+
+// Launch geometry of the one-cell-per-wave kernels: chunks of at most 2^30
+// cells, each a grid that is a multiple of 8.
+constexpr long WAVE_CHUNK_CELLS = 1L << 30;
+inline unsigned wave_chunk_grid(long ncells, long base) {
+  const long n = ncells - base < WAVE_CHUNK_CELLS ? ncells - base : WAVE_CHUNK_CELLS;
+  return (unsigned)((n + 7) / 8 * 8);
 }
 
 // ---------------------------------------------------------------------------
@@ -527,15 +543,16 @@ template <class W>
 int launch_wave_integrate(const DeviceArgs& a, hipStream_t s) {
   if (a.ncells <= 0) return 0;
   const size_t lds = sizeof(double) * W::LDS_DOUBLES;
-  const long cells = a.ncells < (1L << 30) ? a.ncells : (1L << 30);
-  const unsigned grid = (unsigned)((cells + 7) / 8 * 8);  // multiple of 8: see the cell map in the kernel
-  switch (a.method) {
-    case ROS2: wave_integrate_kernel<W, TabRos2><<<grid, 64, lds, s>>>(a); break;
-    case ROS3: wave_integrate_kernel<W, TabRos3><<<grid, 64, lds, s>>>(a); break;
-    case ROS4: wave_integrate_kernel<W, TabRos4><<<grid, 64, lds, s>>>(a); break;
-    case RODAS3: wave_integrate_kernel<W, TabRodas3><<<grid, 64, lds, s>>>(a); break;
-    case RODAS4: wave_integrate_kernel<W, TabRodas4><<<grid, 64, lds, s>>>(a); break;
-    default: return -1;
+  if (a.method < ROS2 || a.method > RODAS4) return -1;
+  for (long base = 0; base < a.ncells; base += WAVE_CHUNK_CELLS) {
+    const unsigned grid = wave_chunk_grid(a.ncells, base);
+    switch (a.method) {
+      case ROS2: wave_integrate_kernel<W, TabRos2><<<grid, 64, lds, s>>>(a, base); break;
+      case ROS3: wave_integrate_kernel<W, TabRos3><<<grid, 64, lds, s>>>(a, base); break;
+      case ROS4: wave_integrate_kernel<W, TabRos4><<<grid, 64, lds, s>>>(a, base); break;
+      case RODAS3: wave_integrate_kernel<W, TabRodas3><<<grid, 64, lds, s>>>(a, base); break;
+      case RODAS4: wave_integrate_kernel<W, TabRodas4><<<grid, 64, lds, s>>>(a, base); break;
+    }
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
