@@ -47,9 +47,18 @@ def bundled_mechanisms() -> list[str]:
     return [os.path.basename(p)[len("mech_"):-len(".hip")] for p in sorted(glob.glob(os.path.join(CSRC, "gen", "mech_*.hip")))]
 
 
+# The persistent batch kernel loops over work items.  Machine-level loop
+# invariant code motion hoists the math library's polynomial constants out of
+# that loop, which pushes the kernel past 256 VGPRs and spills them to scratch
+# once per item; without it the batch kernel has no VGPR spills.  The
+# one-cell-per-wave kernels have no such loop and keep the pass.
+BATCH_FLAGS = ["-mllvm", "-disable-machine-licm"]
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", src, "-o", obj]
+    extra = BATCH_FLAGS if os.path.basename(src).startswith("batch_") else []
+    cmd = [HIPCC, *HIP_FLAGS, *extra, "-x", "hip", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")
