@@ -112,31 +112,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
       if (valid[b]) ylds[rowv[b]] = y[b];
   };
 
-  auto eval_rates = [&](double (&f)[NB]) {
-#pragma unroll
-    for (int p = 0; p < NGP; ++p) {
-      const int gi = p * 64 + lane;
-      if (gi < NG) {
-        const uint8_t* fa = &T_::GFAC[gi * 3];
-        scr[gi] = kreg[p] * (((1.0 * ylds[fa[0]]) * ylds[fa[1]]) * ylds[fa[2]]);
-      }
-    }
-    if (lane == 0) scr[NG] = 0.0;
-    for (int r = lane; r <= NV; r += 64) facc[r] = 0.0;
-#pragma unroll
-    for (int p = 0; p < C_::NFP; ++p) {
-      double acc = 0.0;
-#pragma unroll
-      for (int q = 0; q < C_::TF; ++q) {
-        const uint32_t tm = T_::CF_TERM[(p * C_::TF + q) * 64 + lane];
-        acc = fma(T_::COEF[tm >> 16], scr[tm & 0xFFFFu], acc);
-      }
-      const int row = T_::CF_ROW[p * 64 + lane];
-      if (row != 255) facc[row] = facc[row] + acc;
-    }
-#pragma unroll
-    for (int b = 0; b < NB; ++b) f[b] = valid[b] ? facc[rowv[b]] : 0.0;
-  };
+  auto eval_rates = [&](double (&f)[NB]) { wave_eval_f<W>(ylds, scr, kreg, valid, f, lane); };
 
   // reactant powers of group gi at the state in ylds: pw = y^q, pm1 = q y^(q-1),
   // pm2 = q (q-1) y^(q-2) (zero where the power is too small)

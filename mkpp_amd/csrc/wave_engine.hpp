@@ -1,8 +1,8 @@
 // Wave-per-cell Rosenbrock engine for mechanisms too large for one lane
 // (saprc99: 78 variable species, 1547 LU entries, 9654 products per factor).
 //
-// One 64-lane wavefront integrates one cell; a grid-stride loop hands the
-// wavefront its next cell.  LDS holds the W/LU matrix in row-displacement
+// One 64-lane wavefront integrates one cell (one cell per workgroup, so the
+// hardware dispatcher balances uneven cells).  LDS holds the W/LU matrix in row-displacement
 // storage (element (i, j) at R[i] + j), the stage state y for F gathers and a
 // scratch area for fluxes / Jacobian partials.  Stage vectors live in
 // registers, lane = row (bank b covers rows 64b .. 64b+63).
@@ -26,6 +26,28 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
   return __hiloint2double(hi, lo);
+}
+
+// Constant lane masks of the generated straight-line code, as SGPR pairs that
+// v_cndmask and s_and_saveexec take directly (no per-lane shift and compare).
+// The two moves sit in an asm statement whose input is an SGPR the caller
+// launders once per call, so they stay next to their use inside the step loop
+// rather than being hoisted out of it (hundreds of live masks would spill);
+// ID makes every site distinct, so equal masks are not merged into one long
+// live range.  The statement is not volatile: it orders nothing, and the
+// loads around it still run ahead of the substitution chain.
+template <uint32_t LO, uint32_t HI, int ID>
+__device__ __forceinline__ uint64_t lane_mask(int opaque) {
+  uint32_t lo, hi;
+  asm("s_mov_b32 %0, %2\n\ts_mov_b32 %1, %3 ; mask %4" : "=s"(lo), "=s"(hi) : "i"(LO), "i"(HI), "i"(ID), "s"(opaque));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// v in the lanes of m, +0.0 elsewhere.  The empty asm hides that v is a load,
+// so the select is not turned back into a branch around a predicated load.
+__device__ __forceinline__ double lane_keep(uint64_t m, double v) {
+  asm("" : "+v"(v));
+  return __builtin_amdgcn_inverse_ballot_w64(m) ? v : 0.0;
 }
 
 // Rate constant of one flux group (src/mkpp/lowering.py:199-313); P holds the
@@ -80,6 +102,56 @@ __device__ __forceinline__ double bank(const double (&v)[NB], int row) {
   }
 }
 
+// F(y) by row slot from the state in ylds (rows, fixed species, ONE), in
+// three steps over the scratch area:
+// 1. lanes take flux groups: scr[g] = k_g * y_a * y_b * y_c, scr[NG] = 0.0;
+// 2. each row's terms (coefficient x flux, in the row's order) are cut into
+//    chunks of at most TF; lane j % 64 of pass j / 64 sums chunk j into the
+//    partial slot scr[PBASE + j] (every lane writes; padding adds 0 x 0.0);
+// 3. the lane of each row adds its chunks' partials in chunk order.
+// A row thus sums its terms in the reference's order (lowering.py, F_i over
+// the flux groups) grouped by chunks, a difference of rounding only.  No
+// loop bound depends on data, so every table load and LDS gather of a step
+// can be issued at once.
+template <class W, int NB, int NGP>
+__device__ __forceinline__ void wave_eval_f(const double* __restrict__ ylds, double* __restrict__ scr,
+                                            const double (&kreg)[NGP], const bool (&valid)[NB], double (&f)[NB],
+                                            int lane) {
+  using T_ = WaveTables<W>;
+  using C_ = typename T_::Code;
+  constexpr int NG = W::NG;
+  static_assert(C_::PBASE + C_::NFP * 64 <= W::NSCR + W::NV + 1, "F partial slots exceed the scratch and row-sum areas");
+#pragma unroll
+  for (int p = 0; p < NGP; ++p) {
+    const int g = p * 64 + lane;
+    if (g < NG) {
+      const uint8_t* fa = &T_::GFAC[g * 3];
+      scr[g] = kreg[p] * (((1.0 * ylds[fa[0]]) * ylds[fa[1]]) * ylds[fa[2]]);
+    }
+  }
+  if (lane == 0) scr[NG] = 0.0;
+#pragma unroll
+  for (int p = 0; p < C_::NFP; ++p) {
+    const uint32_t* tw = &T_::CF_TERM[(p * 64 + lane) * (C_::TF / 2)];
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < C_::TF / 2; ++q) {
+      const uint32_t two = tw[q];
+      acc = fma(T_::COEF[(two >> 8) & 0xFFu], scr[two & 0xFFu], acc);
+      acc = fma(T_::COEF[two >> 24], scr[(two >> 16) & 0xFFu], acc);
+    }
+    scr[C_::PBASE + p * 64 + lane] = acc;
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const uint16_t* pp = &T_::CF_PART[(b * 64 + lane) * C_::FMAXCP];
+    double sum = scr[pp[0]];
+#pragma unroll
+    for (int c = 1; c < C_::FMAXC[b]; ++c) sum = sum + scr[pp[c]];
+    f[b] = valid[b] ? sum : 0.0;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // The fused integrate kernel.
 //
@@ -87,7 +159,7 @@ __device__ __forceinline__ double bank(const double (&v)[NB], int row) {
 //   lu   [SPAN + 1]  W / LU factors, row displacement; slot SPAN absorbs padding
 //   ylds [NY]        stage state gathered by the flux lanes (+ fixed species, 1.0)
 //   scr  [NSCR]      fluxes (F) or Jacobian partials (W); last used slot = 0.0
-//   facc [NV + 1]    F row accumulators; slot NV absorbs padding
+//   facc [NV + 1]    with the end of scr: F chunk partials (wave_eval_f)
 // ---------------------------------------------------------------------------
 // MODE 0: the plain integrator.  MODE 1: the same, recording the state and
 // step size of every accepted step (adj->ckpt, adj->hs, adj->nsteps).  MODE 2:
@@ -162,33 +234,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
     }
   }
 
-  // F at the state in ylds: fluxes by group lanes, then fixed-size term
-  // chunks accumulated per row (chunks of one row sit in successive passes)
-  auto eval_rates = [&](double (&f)[NB]) {
-#pragma unroll
-    for (int p = 0; p < NGP; ++p) {
-      const int g = p * 64 + lane;
-      if (g < NG) {
-        const uint8_t* fa = &T_::GFAC[g * 3];
-        scr[g] = kreg[p] * (((1.0 * ylds[fa[0]]) * ylds[fa[1]]) * ylds[fa[2]]);
-      }
-    }
-    if (lane == 0) scr[NG] = 0.0;
-    for (int r = lane; r <= NV; r += 64) facc[r] = 0.0;
-#pragma unroll
-    for (int p = 0; p < C_::NFP; ++p) {
-      double acc = 0.0;
-#pragma unroll
-      for (int q = 0; q < C_::TF; ++q) {
-        const uint32_t tm = T_::CF_TERM[(p * C_::TF + q) * 64 + lane];
-        acc = fma(T_::COEF[tm >> 16], scr[tm & 0xFFFFu], acc);
-      }
-      const int row = T_::CF_ROW[p * 64 + lane];
-      if (row != 255) facc[row] = facc[row] + acc;
-    }
-#pragma unroll
-    for (int b = 0; b < NB; ++b) f[b] = valid[b] ? facc[rowv[b]] : 0.0;
-  };
+  auto eval_rates = [&](double (&f)[NB]) { wave_eval_f<W>(ylds, scr, kreg, valid, f, lane); };
 
   // W = I/(g h) - J(y) into the LU storage
   auto assemble_w = [&](double inv_g_dt) {
@@ -610,7 +656,7 @@ int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
     static constexpr const uint16_t* GDEPOFS = gen::NAME##_wave::GDEPOFS;                                   \
     static constexpr const uint8_t* GDEP = gen::NAME##_wave::GDEP;                                          \
     static constexpr const uint32_t* CF_TERM = gen::NAME##_wave::CF_TERM;                                   \
-    static constexpr const uint8_t* CF_ROW = gen::NAME##_wave::CF_ROW;                                      \
+    static constexpr const uint16_t* CF_PART = gen::NAME##_wave::CF_PART;                                    \
     static constexpr const uint32_t* CW_TERM = gen::NAME##_wave::CW_TERM;                                   \
     static constexpr const uint16_t* CW_ADDR = gen::NAME##_wave::CW_ADDR;                                   \
     static constexpr const uint16_t* GNVOFS = gen::NAME##_wave::GNVOFS;                                     \

@@ -43,7 +43,7 @@ def main(argv=None):
             else:
                 emit_wave.write(mech, p, outdir)
                 t = emit_wave.build_tables(mech, p)
-                lds = (t["span"] + 1) + t["ny"] + (max(t["ng"], t["npart"]) + 1) + (t["nv"] + 1)
+                lds = t["lds_doubles"]
                 engines[name] = (name, "wave", lds)
         if not args.no_hip and name in BUNDLED:
             compiled.append(name)

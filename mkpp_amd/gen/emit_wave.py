@@ -25,6 +25,11 @@ from .plan import LAW_IDS
 MAX_FACTORS = 3
 
 
+# Terms per chunk of the two-level F sum (build_f_tables); a row of n terms
+# takes ceil(n / F_CHUNK_TERMS) partial slots.
+F_CHUNK_TERMS = 8
+
+
 def row_displacement(nv: int, pattern: list[list[int]]) -> tuple[list[int], int]:
     """Greedy first-fit row displacement: returns (R, span)."""
     used: set[int] = set()
@@ -239,7 +244,15 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
         spofs.append(len(spi))
     if len(coefs) > 255:
         raise ValueError("more than 255 distinct stoichiometric coefficients")
+    # scratch: fluxes + a 0.0 slot, or W partials + a 0.0 slot; with the F
+    # row-sum area behind it, it also holds one partial slot per lane and pass
+    # of the chunked F sums (build_f_tables)
+    f_chunks = sum(-(-(fofs[v + 1] - fofs[v]) // F_CHUNK_TERMS) for v in range(nv))
+    f_passes = (f_chunks + 63) // 64
+    nscr = max(ng + 1, npart + 1, ng + 1 + 64 * f_passes - (nv + 1))
+    lds_doubles = (span + 1) + (nv + nfix + 1) + nscr + (nv + 1)
     return {
+        "nscr": nscr, "lds_doubles": lds_doubles,
         "unit": unit, "gpows_phys": gpows_phys, "gnuofs": gnuofs, "gnus": gnus, "gnuc": gnuc,
         "gdepofs": gdepofs, "gdep": gdep, "gnvofs": gnvofs, "gnvr": gnvr, "gnvc": gnvc, "spofs": spofs, "spi": spi,
         "name": p["name"], "n": n, "nv": nv, "nf": nfix, "ng": ng, "nph": p["num_runtime_parameters"],
@@ -261,6 +274,12 @@ def _arr(ctype: str, name: str, vals) -> str:
         vals = [0]
     body = ", ".join(repr(float(v)) if ctype == "double" else str(int(v)) for v in vals)
     return f"__device__ const {ctype} {name}[{len(vals)}] = {{{body}}};"
+
+
+def _arr_aligned(ctype: str, name: str, vals, align: int) -> str:
+    """_arr with an alignment, so that a lane's consecutive entries load as one
+    wide access."""
+    return _arr(ctype, name, vals).replace(f" {name}[", f" __attribute__((aligned({align}))) {name}[", 1)
 
 
 def emit(t: dict) -> str:
@@ -322,7 +341,7 @@ def emit(t: dict) -> str:
         f"  static constexpr int NPART = {t['npart']}, ONE = {t['one']}, NENTRIES = {t['nentries']};",
         f"  static constexpr int THIRD_BODY = {t['third_body']}, NUM_REACTIONS = {t['num_reactions']};",
         f"  static constexpr bool SUPPORTS_RUNTIME_ENVIRONMENT = {'true' if t['supports_runtime_environment'] else 'false'};",
-        "  static constexpr int NSCR = (NG > NPART ? NG : NPART) + 1;  // + a 0.0 padding slot",
+        f"  static constexpr int NSCR = {t['nscr']};  // fluxes or partials + a 0.0 slot; with the row sums: F chunk partials",
         "  static constexpr int LDS_DOUBLES = (SPAN + 1) + NY + NSCR + (NV + 1);",
         "  static constexpr const char* SPECIES[N] = {" + ", ".join(f'"{s}"' for s in t["species"]) + "};",
         "};",
@@ -385,6 +404,33 @@ def _lane_test(mask: int, var: str = "lane") -> str:
     return f"((0x{mask:016x}ull >> {var}) & 1ull)"
 
 
+class _MaskSites:
+    """C++ expressions for constant lane masks as SGPR pairs (`lane_mask` in
+    wave_engine.hpp), one distinct site id per use."""
+
+    def __init__(self):
+        self.n = 0
+
+    def mask(self, m: int) -> str:
+        self.n += 1
+        return f"lane_mask<0x{m & 0xFFFFFFFF:08x}u, 0x{m >> 32:08x}u, {self.n}>(zs)"
+
+    def keep(self, m: int, expr: str) -> str:
+        """expr in the lanes of m, 0 elsewhere."""
+        if m == (1 << 64) - 1:
+            return expr
+        return f"lane_keep({self.mask(m)}, {expr})"
+
+    def test(self, m: int) -> str:
+        """Lane predicate of m."""
+        if m == (1 << 64) - 1:
+            return "true"
+        return f"__builtin_amdgcn_inverse_ballot_w64({self.mask(m)})"
+
+
+_OPAQUE_SGPR = "    int zs = 0;  // laundered once per call: see lane_mask\n    asm volatile(\"\" : \"+s\"(zs));"
+
+
 def _pack_chunks(items: list, tmax: int, width: int = 64):
     """Split each (key, terms) item into chunks of <= tmax terms and place the
     chunks into passes x lanes so that chunks of one key land in strictly
@@ -409,16 +455,58 @@ def _pack_chunks(items: list, tmax: int, width: int = 64):
     return passes
 
 
-def build_code_tables(t: dict, tf: int = 4, tw: int = 2) -> dict:
-    nv = t["nv"]
+def build_f_tables(t: dict, cidx: dict, zero_c: int, tf: int) -> dict:
+    """Two-level F tables.  Each row's terms (flux group, coefficient), in the
+    row's order, are cut into chunks of at most tf terms; chunk j is summed by
+    lane j % 64 in pass j // 64 into the partial slot PBASE + j of the scratch
+    area, and the lane of each row then adds its chunks' partials in order.
+
+    CF_TERM [pass][lane][tf/2] uint32: two terms a word (low half first), a
+      term = flux slot | coefficient index << 8; padding is the zero flux slot
+      NG with coefficient 0.0.
+    CF_PART [bank][lane][maxcp] uint16: the row's partial slots, padded with
+      the zero flux slot NG; maxc[bank] entries are read."""
+    nv, ng, nb = t["nv"], t["ng"], t["nb"]
     coefs = t["coefs"]
-    # F: chunks of row terms (group index, coefficient)
-    fitems = []
+    assert tf % 2 == 0 and ng < 256 and len(coefs) <= 256
+    chunks = []  # (row, [(group, coef)])
     for v in range(nv):
         terms = [(t["fg"][k], coefs[t["fc"][k]]) for k in range(t["fofs"][v], t["fofs"][v + 1])]
-        if terms:
-            fitems.append((v, terms))
-    fpasses = _pack_chunks(fitems, tf)
+        for i in range(0, len(terms), tf):
+            chunks.append((v, terms[i:i + tf]))
+    npass = (len(chunks) + 63) // 64
+    pbase = ng + 1
+    words = []
+    for p in range(npass):
+        for lane in range(64):
+            j = p * 64 + lane
+            terms = chunks[j][1] if j < len(chunks) else []
+            code = [g | (cidx[c] << 8) for g, c in terms] + [ng | (zero_c << 8)] * (tf - len(terms))
+            for q in range(0, tf, 2):
+                words.append(code[q] | (code[q + 1] << 16))
+    parts = {v: [] for v in range(nv)}
+    for j, (v, _) in enumerate(chunks):
+        parts[v].append(pbase + j)
+    slot = t["slot"]
+    maxc = [0] * nb
+    for v in range(nv):
+        maxc[slot[v] // 64] = max(maxc[slot[v] // 64], len(parts[v]))
+    maxc = [max(m, 1) for m in maxc]
+    maxcp = (max(maxc) + 3) // 4 * 4
+    rowof = {slot[v]: v for v in range(nv)}
+    part = []
+    for b in range(nb):
+        for lane in range(64):
+            v = rowof.get(b * 64 + lane)
+            ps = parts[v] if v is not None else []
+            part += ps + [ng] * (maxcp - len(ps))
+    return {"tf": tf, "nfp": npass, "pbase": pbase, "fterm": words, "fpart": part, "fmaxc": maxc,
+            "fmaxcp": maxcp, "fslots": pbase + len(chunks)}
+
+
+def build_code_tables(t: dict, tf: int = F_CHUNK_TERMS, tw: int = 2) -> dict:
+    nv = t["nv"]
+    coefs = t["coefs"]
     # W: chunks of entry terms (partial index, coefficient) keyed by LU address
     witems = []
     for e, addr in enumerate(t["waddr"]):
@@ -426,7 +514,6 @@ def build_code_tables(t: dict, tf: int = 4, tw: int = 2) -> dict:
         if terms:
             witems.append((addr, terms))
     wpasses = _pack_chunks(witems, tw)
-    ZF = t["ng"]  # scratch slot holding 0.0 for padding (fluxes)
     ZW = t["npart"]  # scratch slot holding 0.0 for padding (partials)
 
     cidx = {c: i for i, c in enumerate(coefs)}
@@ -450,7 +537,7 @@ def build_code_tables(t: dict, tf: int = 4, tw: int = 2) -> dict:
                 key.append(ps[lane][0] if lane < len(ps) else key_none)
         return term, key
 
-    fterm, frow = lay(fpasses, tf, ZF, 255)
+    ft = build_f_tables(t, cidx, zero_c, tf)
     wterm, waddr = lay(wpasses, tw, ZW, 0xFFFF)
     # adjoint scatter: chunks of partial slots per species slot (variable and
     # fixed), padded with the zero partial slot
@@ -489,7 +576,7 @@ def build_code_tables(t: dict, tf: int = 4, tw: int = 2) -> dict:
         for lane in range(64):
             ggrp.append(ps[lane][0] if lane < len(ps) else 255)
     assert t["nv"] < 256 and len(coefs) <= 256
-    return {"tf": tf, "tw": tw, "nfp": len(fpasses), "nwp": len(wpasses), "fterm": fterm, "frow": frow,
+    return {**ft, "tw": tw, "nwp": len(wpasses),
             "wterm": wterm, "wkey": waddr, "ts": ts, "nsp": len(spasses), "sterm": sterm, "srow": srow,
             "tg": tg, "ngat": len(gpasses), "gterm": gterm, "ggrp": ggrp}
 
@@ -522,8 +609,8 @@ def emit_code(t: dict, ct: dict) -> str:
     w("")
     w(f"namespace mkpp {{ namespace gen {{ namespace {name}_wave {{")
     w("")
-    w(_arr("uint32_t", "CF_TERM", ct["fterm"]))
-    w(_arr("uint8_t", "CF_ROW", ct["frow"]))
+    w(_arr_aligned("uint32_t", "CF_TERM", ct["fterm"], 16))
+    w(_arr_aligned("uint16_t", "CF_PART", ct["fpart"], 8))
     w(_arr("uint32_t", "CW_TERM", ct["wterm"]))
     w(_arr("uint16_t", "CW_ADDR", ct["wkey"]))
     w(_arr("uint16_t", "CS_TERM", ct["sterm"]))
@@ -533,6 +620,9 @@ def emit_code(t: dict, ct: dict) -> str:
     w("")
     w("struct Code {")
     w(f"  static constexpr int TF = {ct['tf']}, NFP = {ct['nfp']}, TW = {ct['tw']}, NWP = {ct['nwp']};")
+    w(f"  // F partial slots scr[PBASE, PBASE + chunks); a row reads FMAXC[bank] of its FMAXCP-wide list")
+    w(f"  static constexpr int PBASE = {ct['pbase']}, FSLOTS = {ct['fslots']}, FMAXCP = {ct['fmaxcp']};")
+    w(f"  static constexpr int FMAXC[{nb}] = {{{', '.join(str(m) for m in ct['fmaxc'])}}};")
     w(f"  static constexpr int TS = {ct['ts']}, NSP = {ct['nsp']};  // adjoint scatter chunks")
     w(f"  static constexpr int TG = {ct['tg']}, NGAT = {ct['ngat']};  // adjoint gather chunks")
     w("")
@@ -546,6 +636,8 @@ def emit_code(t: dict, ct: dict) -> str:
     w("    int zero = 0;")
     w("    asm volatile(\"\" : \"+v\"(zero));")
     w("    const double* __restrict__ pv = lu + zero;")
+    w(_OPAQUE_SGPR)
+    ms = _MaskSites()
     k0 = t["k0"]
     for k in range(k0):
         rows = t["lrow"][t["lofs"][k]:t["lofs"][k + 1]]
@@ -561,7 +653,7 @@ def emit_code(t: dict, ct: dict) -> str:
             m = fm[b]
             if not m:
                 continue
-            w(f"      if ({_lane_test(m)}) {{")
+            w(f"      if ({ms.test(m)}) {{")
             w(f"        double* r = lu + rb[{b}];")
             w(f"        const double l = r[{k}] / d;")
             w(f"        r[{k}] = l;")
@@ -579,7 +671,7 @@ def emit_code(t: dict, ct: dict) -> str:
         w("    // every update is an unconditional fma that leaves other lanes unchanged.")
         for c in range(k0, nv):
             m = masks_of([r for r in range(k0, nv) if (r, c) in pattern])[0]
-            w(f"    double c{c} = lu[rb[0] + {c}] * ({_lane_test(m)} ? 1.0 : 0.0);")
+            w(f"    double c{c} = {ms.keep(m, f'lu[rb[0] + {c}]')};")
         for k in range(k0, nv):
             rows = t["lrow"][t["lofs"][k]:t["lofs"][k + 1]]
             if not rows:
@@ -588,7 +680,7 @@ def emit_code(t: dict, ct: dict) -> str:
             m = masks_of(rows)[0]
             w(f"    {{  // pivot {k}: {len(rows)} rows x {len(cols)} columns")
             w(f"      const double d = readlane_d(c{k}, {slot[k]});")
-            w(f"      const bool in_l = {_lane_test(m)};")
+            w(f"      const bool in_l = {ms.test(m)};")
             w(f"      const double l = in_l ? c{k} / d : 0.0;")
             w(f"      c{k} = in_l ? l : c{k};")
             for j in cols:
@@ -596,7 +688,7 @@ def emit_code(t: dict, ct: dict) -> str:
             w("    }")
         for c in range(k0, nv):
             m = masks_of([r for r in range(k0, nv) if (r, c) in pattern])[0]
-            w(f"    if ({_lane_test(m)}) lu[rb[0] + {c}] = c{c};")
+            w(f"    if ({ms.test(m)}) lu[rb[0] + {c}] = c{c};")
     w("  }")
     w("")
     # ---- forward and backward substitution ----
@@ -605,7 +697,7 @@ def emit_code(t: dict, ct: dict) -> str:
     # and update every lane: fma(-0, x, w) = w for finite x, and the loads and
     # scalings, which depend on nothing, run ahead of the substitution chain.  Every such address, R[row] + k or
     # R[k] + row with row, k < NV, lies inside the wavefront's LDS block.
-    lds_doubles = (t["span"] + 1) + t["ny"] + (max(t["ng"], t["npart"]) + 1) + (nv + 1)
+    lds_doubles = t["lds_doubles"]
     assert min(R) >= 0 and max(R) + nv - 1 < lds_doubles
     w("  // Solves: every lane loads its entry (in bounds for any row and column),")
     w("  // zeroed outside the column's rows by a factor off the substitution chain.")
@@ -627,16 +719,21 @@ def emit_code(t: dict, ct: dict) -> str:
     w("  // Backward substitution, column oriented: x_k = w_k * (1 / U(k,k)), then rows")
     w("  // of U(:, k) above the diagonal subtract U(i,k) x_k.")
     w("  static __device__ __forceinline__ void backward(const double* __restrict__ lu, const int (&rb)[" + str(nb) + "], double (&wv)[" + str(nb) + "], const double (&rdiag)[" + str(nb) + "], double (&x)[" + str(nb) + "], int lane) {")
+    w("    // w_k is final once column k is reached (later columns update rows above")
+    w("    // it only), so x = w * (1 / U(k,k)) is formed for every row at the end.")
     for k in reversed(range(nv)):
         masks = masks_of([i for (i, kk) in Uset if kk == k and i < k])
+        if not any(masks):
+            continue
         bk, lk = slot[k] // 64, slot[k] % 64
         w(f"    {{ const double xk = readlane_d(wv[{bk}] * rdiag[{bk}], {lk});")
-        w(f"      x[{bk}] = (lane == {lk}) ? xk : x[{bk}];")
         for b in range(nb):
             if masks[b]:
                 w(f"      {{ const double u = lu[rb[{b}] + {k}] * ({_lane_test(masks[b])} ? 1.0 : 0.0);")
                 w(f"        wv[{b}] = fma(-u, xk, wv[{b}]); }}")
         w("    }")
+    for b in range(nb):
+        w(f"    x[{b}] = wv[{b}] * rdiag[{b}];")
     w("  }")
     w("")
     # ---- transposed solves (adjoint): W^T = U^T L^T ----
@@ -644,16 +741,21 @@ def emit_code(t: dict, ct: dict) -> str:
     w("  // is reached, then every j > k in row k of U subtracts U(k,j) z_k.  The lane")
     w("  // holding row j (rv) reads U(k,j) at R[k] + j.")
     w("  static __device__ __forceinline__ void tforward(const double* __restrict__ lu, const int (&rv)[" + str(nb) + "], double (&b)[" + str(nb) + "], const double (&rdiag)[" + str(nb) + "], double (&z)[" + str(nb) + "], int lane) {")
+    w("    // b_k is final once column k is reached, so z = b * (1 / U(k,k)) is formed")
+    w("    // for every row at the end.")
     for k in range(nv):
         masks = masks_of([j for (kk, j) in Uset if kk == k and j > k])
+        if not any(masks):
+            continue
         bk, lk = slot[k] // 64, slot[k] % 64
         w(f"    {{ const double zk = readlane_d(b[{bk}] * rdiag[{bk}], {lk});")
-        w(f"      z[{bk}] = (lane == {lk}) ? zk : z[{bk}];")
         for bb in range(nb):
             if masks[bb]:
                 w(f"      {{ const double u = lu[{R[k]} + rv[{bb}]] * ({_lane_test(masks[bb])} ? 1.0 : 0.0);")
                 w(f"        b[{bb}] = fma(-u, zk, b[{bb}]); }}")
         w("    }")
+    for bb in range(nb):
+        w(f"    z[{bb}] = b[{bb}] * rdiag[{bb}];")
     w("  }")
     w("")
     w("  // L^T x = z in place (unit diagonal): x_k = z_k when column k is reached,")

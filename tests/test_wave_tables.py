@@ -191,6 +191,66 @@ def test_emulated_wave_engine_matches_oracle(case):
     np.testing.assert_allclose(x, xo[lg_of_v], rtol=1e-9, atol=1e-300)
 
 
+def test_chunked_f_tables_match_oracle(case):
+    """CF_TERM / CF_PART decoded as wave_eval_f decodes them: chunk partials
+    into scratch slots, then each row's partials in chunk order."""
+    import copy
+
+    name, mech, p, t = case
+    t = copy.deepcopy(t)
+    ct = emit_wave.build_code_tables(t)  # may append 0.0 to t["coefs"]
+    o = Oracle(name)
+    rng = np.random.default_rng(11)
+    n, nv, ng, nb = t["n"], t["nv"], t["ng"], t["nb"]
+    tf, nfp, pbase, maxcp = ct["tf"], ct["nfp"], ct["pbase"], ct["fmaxcp"]
+    nscr = t["nscr"]
+    assert nscr >= max(ng, t["npart"]) + 1
+    # bounds: every written partial slot inside the scratch and row-sum areas,
+    # every term inside the flux slots and the coefficient table
+    assert pbase == ng + 1 and pbase + 64 * nfp <= nscr + nv + 1
+    assert len(ct["fterm"]) == nfp * 64 * (tf // 2) and len(ct["fpart"]) == nb * 64 * maxcp
+    terms = [(w >> s) & 0xFFFF for w in ct["fterm"] for s in (0, 16)]
+    assert max(x & 0xFF for x in terms) <= ng and max(x >> 8 for x in terms) < len(t["coefs"])
+    assert all(ng <= i < pbase + 64 * nfp for i in ct["fpart"])
+    assert all(1 <= m <= maxcp for m in ct["fmaxc"])
+    y = 10.0 ** rng.uniform(7, 12, n)
+    tb = t["third_body"]
+    M = y[tb] if tb >= 0 else 2.4e19
+    T = 291.0
+    jv = rng.uniform(0, 1e-3, max(t["nph"], 1))
+    k = np.array([_rate(t["law"][g], t["params"][g], T, M, jv[t["photo"][g]] if t["law"][g] == 0 else 0.0)
+                  for g in range(ng)])
+    ylds = np.zeros(t["ny"])
+    for v, ph in enumerate(t["var_phys"]):
+        ylds[v] = 1.0 if p["unit_value"][ph] else y[ph]
+    for q, ph in enumerate(t["fixed_phys"]):
+        ylds[nv + q] = y[ph]
+    ylds[t["one"]] = 1.0
+    coef = np.array(t["coefs"])
+    scr = np.full(nscr + nv + 1, np.nan)
+    for g, f in enumerate(t["gfac"]):
+        scr[g] = k[g] * (((1.0 * ylds[f[0]]) * ylds[f[1]]) * ylds[f[2]])
+    scr[ng] = 0.0
+    for pp in range(nfp):
+        for lane in range(64):
+            acc = 0.0
+            for q in range(tf):
+                x = terms[(pp * 64 + lane) * tf + q]
+                acc = acc + coef[x >> 8] * scr[x & 0xFF]
+            scr[pbase + pp * 64 + lane] = acc
+    F = np.zeros(nv)
+    for v in range(nv):
+        b, lane = divmod(t["slot"][v], 64)
+        lst = ct["fpart"][(b * 64 + lane) * maxcp:(b * 64 + lane + 1) * maxcp]
+        s = scr[lst[0]]
+        for c in range(1, ct["fmaxc"][b]):
+            s = s + scr[lst[c]]
+        F[v] = s
+        assert all(i == ng for i in lst[ct["fmaxc"][b]:])  # entries past FMAXC are padding only
+    Fr = o.rates(y, jv, T)
+    np.testing.assert_allclose(F, Fr[t["var_phys"]], rtol=1e-12, atol=1e-300)
+
+
 def test_law_parameter_order():
     assert LAW_PARAMS["TROE"] == ["A0", "B0", "C0", "A1", "B1", "C1", "Fc"]
     assert LAW_PARAMS["EP2"] == ["A0", "C0", "A2", "C2", "A3", "C3"]
