@@ -30,21 +30,45 @@ MAX_FACTORS = 3
 F_CHUNK_TERMS = 8
 
 
-def row_displacement(nv: int, pattern: list[list[int]]) -> tuple[list[int], int]:
-    """Greedy first-fit row displacement: returns (R, span)."""
+def row_displacement(nv: int, pattern: list[list[int]], bank: list[int] | None = None,
+                     modulus: int = 0) -> tuple[list[int], int, list[int]]:
+    """Greedy first-fit row displacement: returns (R, span, lane).
+
+    With a modulus M, each row also takes a lane of its bank (`bank[i]`), and
+    its displacement satisfies R[i] = lane (mod M): the row picks, among its
+    bank's free lanes, the one whose first fitting displacement is smallest.
+    The lanes of any M consecutive lanes then hold rows whose R differ mod M,
+    so an LDS access by every lane of a bank at R[row] + k (the same k in
+    every lane: the factorization and the solves) spreads over distinct
+    banks."""
     used: set[int] = set()
     R = [0] * nv
+    lane = list(range(nv))
+    free = {b: set(range(64)) for b in set(bank or [0])}
     order = sorted(range(nv), key=lambda i: (-len(pattern[i]), i))
     for i in order:
         cols = pattern[i]
-        r = -min(cols)
-        while any((r + j) in used for j in cols):
-            r += 1
-        R[i] = r
+        best = None
+        for ln in (sorted(free[bank[i]]) if modulus else [None]):
+            r = -min(cols)
+            if modulus:
+                r += (ln - r) % modulus
+            while any((r + j) in used for j in cols):
+                r += modulus or 1
+            if best is None or r < best[0]:
+                best = (r, ln)
+        R[i] = best[0]
+        if modulus:
+            lane[i] = best[1]
+            free[bank[i]].discard(best[1])
         for j in cols:
-            used.add(r + j)
+            used.add(R[i] + j)
     span = max(R[i] + max(pattern[i]) for i in range(nv)) + 1
-    return R, span
+    return R, span, lane
+
+
+# LDS doubles per wavefront that keep 8 wavefronts (2 per SIMD) on a CU's 160 KB
+LDS_DOUBLES_8_WAVES = 160 * 1024 // 8 // 8
 
 
 def build_tables(mech: Mechanism, p: dict) -> dict:
@@ -63,7 +87,6 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
     for i in range(nv):
         pattern[i].sort()
         assert i in pattern[i]
-    R, span = row_displacement(nv, pattern)
     Lset, Uset = set(L), set(U)
     # LU schedule: per pivot k the rows of column k below the diagonal and the
     # columns of row k right of the diagonal (right-looking; per-entry product
@@ -103,14 +126,6 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
         ranked = [r for r in sorted(range(nv), key=lambda r: (-part[r], r)) if r < k0]
         bank0 = sorted(forced + ranked[:64 - len(forced)])
         bank1 = sorted(ranked[64 - len(forced):])
-    slot = [0] * nv
-    for lane, r in enumerate(bank0):
-        slot[r] = lane
-    for lane, r in enumerate(bank1):
-        slot[r] = 64 + lane
-    rowof = [255] * (nb * 64)
-    for r in range(nv):
-        rowof[slot[r]] = r
     # solve masks per column k and bank: rows below (forward) / above (backward)
     fmask = [[0] * nb for _ in range(nv)]
     bmask = [[0] * nb for _ in range(nv)]
@@ -199,7 +214,7 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
                 for a, (s, q) in enumerate(g["powers"]):
                     if s == pj:
                         terms.append((poff[gi] + a, cid(float(c))))
-        entries.append((R[vi] + vj, int(vi == vj), terms))
+        entries.append(((vi, vj), int(vi == vj), terms))  # address once R is known
     entries.sort(key=lambda e: -len(e[2]))
     waddr, wdiag, wofs, wpi, wci = [], [], [0], [], []
     for addr, dg, terms in entries:
@@ -250,6 +265,28 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
     f_chunks = sum(-(-(fofs[v + 1] - fofs[v]) // F_CHUNK_TERMS) for v in range(nv))
     f_passes = (f_chunks + 63) // 64
     nscr = max(ng + 1, npart + 1, ng + 1 + 64 * f_passes - (nv + 1))
+    # Displacements and lanes together: R[row] = lane (mod 32) where the LDS
+    # budget of 8 wavefronts per CU allows it, else mod 16, else no constraint.
+    bank = [0] * nv
+    for r in bank1:
+        bank[r] = 1
+    for modulus in (32, 16, 0):
+        R, span, lane_of = row_displacement(nv, pattern, bank, modulus)
+        if not modulus or (span + 1) + (nv + nfix + 1) + nscr + (nv + 1) <= LDS_DOUBLES_8_WAVES:
+            break
+    slot = [0] * nv
+    if modulus:
+        for r in range(nv):
+            slot[r] = 64 * bank[r] + lane_of[r]
+    else:
+        for lane, r in enumerate(bank0):
+            slot[r] = lane
+        for lane, r in enumerate(bank1):
+            slot[r] = 64 + lane
+    rowof = [255] * (nb * 64)
+    for r in range(nv):
+        rowof[slot[r]] = r
+    waddr = [R[vi] + vj for (vi, vj) in waddr]
     lds_doubles = (span + 1) + (nv + nfix + 1) + nscr + (nv + 1)
     return {
         "nscr": nscr, "lds_doubles": lds_doubles,
@@ -387,6 +424,8 @@ def write(mech: Mechanism, p: dict, outdir: str) -> list:
 # Trailing rows of the LU factorized in registers (lanes = rows, one register per
 # column); 35 columns of doubles are 70 VGPRs.
 REGISTER_BLOCK = 35
+# Pivot-row broadcasts of the register block issued together (see factor).
+BROADCAST_GROUP = 4
 
 
 def _lane_test(mask: int, var: str = "lane") -> str:
@@ -635,7 +674,7 @@ def emit_code(t: dict, ct: dict) -> str:
     w("    // compiler hoists every one out of the step loop and spills them")
     w("    int zero = 0;")
     w("    asm volatile(\"\" : \"+v\"(zero));")
-    w("    const double* __restrict__ pv = lu + zero;")
+    w("    const double* pv = lu + zero;  // aliases lu: pivot rows are read after their stores")
     w(_OPAQUE_SGPR)
     ms = _MaskSites()
     k0 = t["k0"]
@@ -683,8 +722,14 @@ def emit_code(t: dict, ct: dict) -> str:
             w(f"      const bool in_l = {ms.test(m)};")
             w(f"      const double l = in_l ? c{k} / d : 0.0;")
             w(f"      c{k} = in_l ? l : c{k};")
-            for j in cols:
-                w(f"      c{j} = fma(-l, readlane_d(c{j}, {slot[k]}), c{j});")
+            # broadcasts in groups: a group's readlanes issue together, ahead of
+            # their FMAs, instead of each waiting out its SGPR hazard
+            for g0 in range(0, len(cols), BROADCAST_GROUP):
+                grp = cols[g0:g0 + BROADCAST_GROUP]
+                for j in grp:
+                    w(f"      const double u{j} = readlane_d(c{j}, {slot[k]});")
+                for j in grp:
+                    w(f"      c{j} = fma(-l, u{j}, c{j});")
             w("    }")
         for c in range(k0, nv):
             m = masks_of([r for r in range(k0, nv) if (r, c) in pattern])[0]
@@ -699,39 +744,53 @@ def emit_code(t: dict, ct: dict) -> str:
     # R[k] + row with row, k < NV, lies inside the wavefront's LDS block.
     lds_doubles = t["lds_doubles"]
     assert min(R) >= 0 and max(R) + nv - 1 < lds_doubles
+    def emit_levels(level, order, masks_fn, value_fn, name, addr_fn, acc):
+        """Column-oriented substitution in level order: the pivots of one level
+        depend only on earlier levels, so their broadcasts (readlane) are all
+        issued before the level's updates, and the substitution chain runs
+        through the levels rather than through every pivot.  Within a level,
+        pivots keep `order`; a row therefore takes its updates in (level, k)
+        order."""
+        piv = [k for k in order if any(masks_fn(k))]
+        for lv in sorted(set(level[k] for k in piv)):
+            ks = [k for k in piv if level[k] == lv]
+            w(f"    {{  // level {lv}: pivots {', '.join(str(k) for k in ks)}")
+            for k in ks:
+                w(f"      const double x{k} = readlane_d({value_fn(k)}, {slot[k] % 64});")
+            for k in ks:
+                masks = masks_fn(k)
+                for b in range(nb):
+                    if masks[b]:
+                        w(f"      {{ const double {name} = {addr_fn(k, b)} * ({_lane_test(masks[b])} ? 1.0 : 0.0);")
+                        w(f"        {acc}[{b}] = fma(-{name}, x{k}, {acc}[{b}]); }}")
+            w("    }")
+
     w("  // Solves: every lane loads its entry (in bounds for any row and column),")
     w("  // zeroed outside the column's rows by a factor off the substitution chain.")
-    w("  // Forward substitution, column oriented: x_k = w_k is final when column k")
-    w("  // is reached; rows of L(:, k) subtract L(i,k) x_k (ascending k per row).")
+    w("  // Forward substitution, column oriented, in level order: x_k = w_k is final")
+    w("  // when its level is reached; rows of L(:, k) subtract L(i,k) x_k.")
     w("  static __device__ __forceinline__ void forward(const double* __restrict__ lu, const int (&rb)[" + str(nb) + "], double (&wv)[" + str(nb) + "], int lane) {")
+    # level schedule: x_k is final once every column j with (k, j) in L is done
+    lev = [0] * nv
     for k in range(nv):
-        masks = masks_of([i for (i, kk) in Lset if kk == k])
-        if not any(masks):
-            continue
-        w(f"    {{ const double xk = readlane_d(wv[{slot[k] // 64}], {slot[k] % 64});")
-        for b in range(nb):
-            if masks[b]:
-                w(f"      {{ const double l = lu[rb[{b}] + {k}] * ({_lane_test(masks[b])} ? 1.0 : 0.0);")
-                w(f"        wv[{b}] = fma(-l, xk, wv[{b}]); }}")
-        w("    }")
+        for i in t["lrow"][t["lofs"][k]:t["lofs"][k + 1]]:
+            lev[i] = max(lev[i], lev[k] + 1)
+    emit_levels(lev, range(nv), lambda k: masks_of([i for (i, kk) in Lset if kk == k]),
+                lambda k: f"wv[{slot[k] // 64}]", "l", lambda k, b: f"lu[rb[{b}] + {k}]", "wv")
     w("  }")
     w("")
-    w("  // Backward substitution, column oriented: x_k = w_k * (1 / U(k,k)), then rows")
-    w("  // of U(:, k) above the diagonal subtract U(i,k) x_k.")
+    w("  // Backward substitution, column oriented, in level order: x_k = w_k / U(k,k)")
+    w("  // (by the reciprocal diagonal), then rows of U(:, k) above it subtract U(i,k) x_k.")
     w("  static __device__ __forceinline__ void backward(const double* __restrict__ lu, const int (&rb)[" + str(nb) + "], double (&wv)[" + str(nb) + "], const double (&rdiag)[" + str(nb) + "], double (&x)[" + str(nb) + "], int lane) {")
     w("    // w_k is final once column k is reached (later columns update rows above")
     w("    // it only), so x = w * (1 / U(k,k)) is formed for every row at the end.")
-    for k in reversed(range(nv)):
-        masks = masks_of([i for (i, kk) in Uset if kk == k and i < k])
-        if not any(masks):
-            continue
-        bk, lk = slot[k] // 64, slot[k] % 64
-        w(f"    {{ const double xk = readlane_d(wv[{bk}] * rdiag[{bk}], {lk});")
-        for b in range(nb):
-            if masks[b]:
-                w(f"      {{ const double u = lu[rb[{b}] + {k}] * ({_lane_test(masks[b])} ? 1.0 : 0.0);")
-                w(f"        wv[{b}] = fma(-u, xk, wv[{b}]); }}")
-        w("    }")
+    # level schedule: x_k is final once every column j > k with (k, j) in U is done
+    levb = [0] * nv
+    for j in reversed(range(nv)):
+        for i in [i for (i, kk) in Uset if kk == j and i < j]:
+            levb[i] = max(levb[i], levb[j] + 1)
+    emit_levels(levb, list(reversed(range(nv))), lambda k: masks_of([i for (i, kk) in Uset if kk == k and i < k]),
+                lambda k: f"wv[{slot[k] // 64}] * rdiag[{slot[k] // 64}]", "u", lambda k, b: f"lu[rb[{b}] + {k}]", "wv")
     for b in range(nb):
         w(f"    x[{b}] = wv[{b}] * rdiag[{b}];")
     w("  }")
