@@ -38,7 +38,7 @@ namespace mkpp {
 template <class W, class Tab>
 constexpr int adjoint_lds_doubles() {
   return W::LDS_DOUBLES + (2 * Tab::S + 1) * (W::NV + 1) + Tab::S * W::NY + (W::NV + 1) + (W::NF + 1) + W::NY +
-         (W::NG + 1);
+         (W::NG + 1) + WaveTables<W>::Code::SEXTRA;
 }
 
 template <class W, class Tab>
@@ -56,7 +56,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
   double* kbv = kv + S * NY;         // [S][NV+1] Kbar_s by row
   double* uv = kbv + S * (NV + 1);   // [NV+1]    u of a J^T pass by row
   double* fbar = uv + (NV + 1);      // [NF]      adjoint of the fixed species
-  double* sacc = fbar + (NF + 1);    // [NY]      scatter sums per species slot
+  double* sacc = fbar + (NF + 1);    // [NY]      with wj (+ SEXTRA): scatter chunk partials
   double* wj = sacc + NY;            // [NG+1]    nu_g . u of a J^T pass
   double* wh = lu;                   // [S+1][NG+1] nu_g . rbv rows (second-derivative pass, after the LU is done)
 
@@ -203,22 +203,41 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
   };
 
   // scr[partial slot] summed per species slot: rows into out[], fixed species
-  // into fbar.  Fixed-trip chunks (CS tables): one chunk of partial slots per
-  // lane and pass, the chunks of one species in successive passes.
+  // into fbar.  Two levels (CS tables): chunks of up to TS partial slots, any
+  // species, summed by lane into cp[chunk]; then each species' lane adds its
+  // chunks in order.  cp spans the species-sum and group-dot areas, whose
+  // contents (the dots) are consumed before the scatter.
   auto scatter = [&](double (&out)[NB]) {
+    double* cp = sacc;
     if (lane == 0) scr[W::NPART] = 0.0;
-    for (int i = lane; i < NV + NF; i += 64) sacc[i] = 0.0;
 #pragma unroll
     for (int p = 0; p < C_::NSP; ++p) {
+      const uint32_t* tw = &T_::CS_TERM[(p * 64 + lane) * (C_::TS / 2)];
       double acc = 0.0;
 #pragma unroll
-      for (int q = 0; q < C_::TS; ++q) acc += scr[T_::CS_TERM[(p * C_::TS + q) * 64 + lane]];
-      const int row = T_::CS_ROW[p * 64 + lane];
-      if (row != 255) sacc[row] += acc;
+      for (int q = 0; q < C_::TS / 2; ++q) {
+        const uint32_t two = tw[q];
+        acc += scr[two & 0xFFFFu];
+        acc += scr[two >> 16];
+      }
+      cp[p * 64 + lane] = acc;
     }
+    if (lane == 0) cp[C_::SZERO] = 0.0;
 #pragma unroll
-    for (int b = 0; b < NB; ++b) out[b] = valid[b] ? sacc[rowv[b]] : 0.0;
-    if (lane < NF) fbar[lane] += sacc[NV + lane];
+    for (int b = 0; b < NB; ++b) {
+      const uint16_t* pp = &T_::CS_PART[(b * 64 + lane) * C_::SMAXCP];
+      double sum = cp[pp[0]];
+#pragma unroll
+      for (int c = 1; c < C_::SMAXC[b]; ++c) sum += cp[pp[c]];
+      out[b] = valid[b] ? sum : 0.0;
+    }
+    if (lane < NF) {
+      const uint16_t* pp = &T_::CS_FIX[lane * C_::SFMAXCP];
+      double sum = cp[pp[0]];
+#pragma unroll
+      for (int c = 1; c < C_::SFMAXC; ++c) sum += cp[pp[c]];
+      fbar[lane] += sum;
+    }
   };
 
   // J(Y)^T u at the state in ylds (u by row in uv), plus kbar += u . dF/dk

@@ -664,8 +664,9 @@ int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
     static constexpr const uint8_t* GNVC = gen::NAME##_wave::GNVC;                                          \
     static constexpr const uint16_t* SPOFS = gen::NAME##_wave::SPOFS;                                       \
     static constexpr const uint16_t* SPI = gen::NAME##_wave::SPI;                                           \
-    static constexpr const uint16_t* CS_TERM = gen::NAME##_wave::CS_TERM;                                   \
-    static constexpr const uint8_t* CS_ROW = gen::NAME##_wave::CS_ROW;                                      \
+    static constexpr const uint32_t* CS_TERM = gen::NAME##_wave::CS_TERM;                                   \
+    static constexpr const uint16_t* CS_PART = gen::NAME##_wave::CS_PART;                                   \
+    static constexpr const uint16_t* CS_FIX = gen::NAME##_wave::CS_FIX;                                     \
     static constexpr const uint16_t* CG_TERM = gen::NAME##_wave::CG_TERM;                                   \
     static constexpr const uint8_t* CG_GRP = gen::NAME##_wave::CG_GRP;                                      \
     using Code = gen::NAME##_wave::Code;                                                                    \

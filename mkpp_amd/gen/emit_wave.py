@@ -543,6 +543,62 @@ def build_f_tables(t: dict, cidx: dict, zero_c: int, tf: int) -> dict:
             "fmaxcp": maxcp, "fslots": pbase + len(chunks)}
 
 
+def build_s_tables(t: dict, ts: int = F_CHUNK_TERMS) -> dict:
+    """Two-level scatter tables of the adjoint's J^T passes: each species
+    slot's partial slots (variable rows, then fixed species) are cut into
+    chunks of at most ts; chunk j is summed by lane j % 64 of pass j // 64 into
+    cp[j], and each species' lane adds its chunks' partials in order.
+
+    CS_TERM [pass][lane][ts/2] uint32: two partial slots a word (low half
+      first); padding is the zero partial slot NPART.
+    CS_PART [bank][lane][smaxcp] uint16: a row's chunk indices, padded with
+      SZERO (the zero chunk slot); CS_FIX [fixed][sfmaxcp]: the same per fixed
+      species (its lane = its index)."""
+    nv, nf, nb, npart = t["nv"], t["nf"], t["nb"], t["npart"]
+    assert ts % 2 == 0 and npart < 65536
+    chunks = []
+    for l in range(nv + nf):
+        terms = list(t["spi"][t["spofs"][l]:t["spofs"][l + 1]])
+        for i in range(0, len(terms), ts):
+            chunks.append((l, terms[i:i + ts]))
+    npass = max(1, (len(chunks) + 63) // 64)
+    szero = 64 * npass
+    words = []
+    for p in range(npass):
+        for lane in range(64):
+            j = p * 64 + lane
+            terms = chunks[j][1] if j < len(chunks) else []
+            code = terms + [npart] * (ts - len(terms))
+            for q in range(0, ts, 2):
+                words.append(code[q] | (code[q + 1] << 16))
+    parts = {l: [] for l in range(nv + nf)}
+    for j, (l, _) in enumerate(chunks):
+        parts[l].append(j)
+    slot = t["slot"]
+    rowof = {slot[v]: v for v in range(nv)}
+    smaxc = [1] * nb
+    for v in range(nv):
+        smaxc[slot[v] // 64] = max(smaxc[slot[v] // 64], len(parts[v]))
+    smaxcp = (max(smaxc) + 3) // 4 * 4
+    spart = []
+    for b in range(nb):
+        for lane in range(64):
+            v = rowof.get(b * 64 + lane)
+            ps = parts[v] if v is not None else []
+            spart += ps + [szero] * (smaxcp - len(ps))
+    sfmaxc = max([1] + [len(parts[nv + f]) for f in range(nf)])
+    sfmaxcp = sfmaxc
+    sfix = []
+    for f in range(max(nf, 1)):
+        ps = parts[nv + f] if f < nf else []
+        sfix += ps + [szero] * (sfmaxcp - len(ps))
+    # the partials live in the adjoint's species-sum and group-dot areas
+    # (NY + NG + 1 doubles, both free during the scatter) and beyond if needed
+    sextra = max(0, szero + 1 - (t["ny"] + t["ng"] + 1))
+    return {"ts": ts, "nsp": npass, "sterm": words, "spart": spart, "sfix": sfix, "szero": szero,
+            "smaxc": smaxc, "smaxcp": smaxcp, "sfmaxc": sfmaxc, "sfmaxcp": sfmaxcp, "sextra": sextra}
+
+
 def build_code_tables(t: dict, tf: int = F_CHUNK_TERMS, tw: int = 2) -> dict:
     nv = t["nv"]
     coefs = t["coefs"]
@@ -578,22 +634,7 @@ def build_code_tables(t: dict, tf: int = F_CHUNK_TERMS, tw: int = 2) -> dict:
 
     ft = build_f_tables(t, cidx, zero_c, tf)
     wterm, waddr = lay(wpasses, tw, ZW, 0xFFFF)
-    # adjoint scatter: chunks of partial slots per species slot (variable and
-    # fixed), padded with the zero partial slot
-    ts = 4
-    sitems = []
-    for l in range(t["nv"] + t["nf"]):
-        terms = list(t["spi"][t["spofs"][l]:t["spofs"][l + 1]])
-        if terms:
-            sitems.append((l, terms))
-    spasses = _pack_chunks(sitems, ts)
-    sterm, srow = [], []
-    for ps in spasses:
-        for q in range(ts):
-            for lane in range(64):
-                sterm.append(ps[lane][1][q] if lane < len(ps) and q < len(ps[lane][1]) else ZW)
-        for lane in range(64):
-            srow.append(ps[lane][0] if lane < len(ps) else 255)
+    st = build_s_tables(t)
     # adjoint gather nu_g . u: chunks of (row, coefficient) terms per group;
     # padding reads row NV (an always-zero slot) with coefficient 0
     tg = 4
@@ -616,7 +657,7 @@ def build_code_tables(t: dict, tf: int = F_CHUNK_TERMS, tw: int = 2) -> dict:
             ggrp.append(ps[lane][0] if lane < len(ps) else 255)
     assert t["nv"] < 256 and len(coefs) <= 256
     return {**ft, "tw": tw, "nwp": len(wpasses),
-            "wterm": wterm, "wkey": waddr, "ts": ts, "nsp": len(spasses), "sterm": sterm, "srow": srow,
+            "wterm": wterm, "wkey": waddr, **st,
             "tg": tg, "ngat": len(gpasses), "gterm": gterm, "ggrp": ggrp}
 
 
@@ -652,8 +693,9 @@ def emit_code(t: dict, ct: dict) -> str:
     w(_arr_aligned("uint16_t", "CF_PART", ct["fpart"], 8))
     w(_arr("uint32_t", "CW_TERM", ct["wterm"]))
     w(_arr("uint16_t", "CW_ADDR", ct["wkey"]))
-    w(_arr("uint16_t", "CS_TERM", ct["sterm"]))
-    w(_arr("uint8_t", "CS_ROW", ct["srow"]))
+    w(_arr_aligned("uint32_t", "CS_TERM", ct["sterm"], 16))
+    w(_arr_aligned("uint16_t", "CS_PART", ct["spart"], 8))
+    w(_arr("uint16_t", "CS_FIX", ct["sfix"]))
     w(_arr("uint16_t", "CG_TERM", ct["gterm"]))
     w(_arr("uint8_t", "CG_GRP", ct["ggrp"]))
     w("")
@@ -663,6 +705,12 @@ def emit_code(t: dict, ct: dict) -> str:
     w(f"  static constexpr int PBASE = {ct['pbase']}, FSLOTS = {ct['fslots']}, FMAXCP = {ct['fmaxcp']};")
     w(f"  static constexpr int FMAXC[{nb}] = {{{', '.join(str(m) for m in ct['fmaxc'])}}};")
     w(f"  static constexpr int TS = {ct['ts']}, NSP = {ct['nsp']};  // adjoint scatter chunks")
+    w(f"  // scatter chunk partials cp[0, 64 NSP), cp[SZERO] = 0.0; rows read SMAXC[bank] of")
+    w(f"  // their SMAXCP-wide list, fixed species SFMAXC of SFMAXCP; SEXTRA: LDS the")
+    w(f"  // partials need beyond the adjoint's species-sum and group-dot areas")
+    w(f"  static constexpr int SZERO = {ct['szero']}, SMAXCP = {ct['smaxcp']}, SFMAXC = {ct['sfmaxc']}, SFMAXCP = {ct['sfmaxcp']};")
+    w(f"  static constexpr int SMAXC[{nb}] = {{{', '.join(str(m) for m in ct['smaxc'])}}};")
+    w(f"  static constexpr int SEXTRA = {ct['sextra']};")
     w(f"  static constexpr int TG = {ct['tg']}, NGAT = {ct['ngat']};  // adjoint gather chunks")
     w("")
     # ---- factorization ----

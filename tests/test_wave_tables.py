@@ -251,6 +251,48 @@ def test_chunked_f_tables_match_oracle(case):
     np.testing.assert_allclose(F, Fr[t["var_phys"]], rtol=1e-12, atol=1e-300)
 
 
+def test_chunked_scatter_tables(case):
+    """CS_TERM / CS_PART / CS_FIX decoded as the adjoint's scatter decodes
+    them give each species slot the sum of its partial slots (every slot used
+    once), and every index stays inside the areas the kernel gives them."""
+    import copy
+
+    _, _, _, t = case
+    t = copy.deepcopy(t)
+    ct = emit_wave.build_code_tables(t)
+    nv, nf, nb, npart, ng, ny = t["nv"], t["nf"], t["nb"], t["npart"], t["ng"], t["ny"]
+    ts, nsp, szero = ct["ts"], ct["nsp"], ct["szero"]
+    assert szero == 64 * nsp and szero + 1 <= ny + ng + 1 + ct["sextra"]
+    terms = [(w >> s) & 0xFFFF for w in ct["sterm"] for s in (0, 16)]
+    assert len(terms) == nsp * 64 * ts and max(terms) <= npart
+    assert all(0 <= i <= szero for i in ct["spart"] + ct["sfix"])
+    rng = np.random.default_rng(5)
+    scr = np.append(rng.standard_normal(npart), 0.0)
+    cp = np.full(szero + 1, np.nan)
+    for j in range(nsp * 64):
+        acc = 0.0
+        for q in range(ts):
+            acc += scr[terms[j * ts + q]]
+        cp[j] = acc
+    cp[szero] = 0.0
+    want = [sum(scr[i] for i in t["spi"][t["spofs"][l]:t["spofs"][l + 1]]) for l in range(nv + nf)]
+    for v in range(nv):
+        b, lane = divmod(t["slot"][v], 64)
+        lst = ct["spart"][(b * 64 + lane) * ct["smaxcp"]:(b * 64 + lane + 1) * ct["smaxcp"]]
+        assert all(i == szero for i in lst[ct["smaxc"][b]:])
+        got = cp[lst[0]]
+        for c in range(1, ct["smaxc"][b]):
+            got += cp[lst[c]]
+        assert abs(got - want[v]) <= 1e-12 * (1 + abs(want[v]))
+    for f in range(nf):
+        lst = ct["sfix"][f * ct["sfmaxcp"]:(f + 1) * ct["sfmaxcp"]]
+        got = sum(cp[lst[c]] for c in range(ct["sfmaxc"]))
+        assert abs(got - want[nv + f]) <= 1e-12 * (1 + abs(want[nv + f]))
+    # every partial slot reaches exactly one species once
+    used = sorted(i for i in terms if i != npart)
+    assert used == list(range(npart))
+
+
 def test_law_parameter_order():
     assert LAW_PARAMS["TROE"] == ["A0", "B0", "C0", "A1", "B1", "C1", "Fc"]
     assert LAW_PARAMS["EP2"] == ["A0", "C0", "A2", "C2", "A3", "C3"]
