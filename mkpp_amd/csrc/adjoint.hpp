@@ -158,7 +158,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
       }
     }
     if (lane == 0) scr[W::NPART] = 0.0;
-    for (int i = lane; i <= W::SPAN; i += 64) lu[i] = 0.0;
+    wave_zero_lu<W>(lu, lane);
 #pragma unroll
     for (int b = 0; b < NB; ++b)
       if (valid[b]) lu[rb[b] + rowv[b]] = inv_g_dt;
@@ -170,8 +170,10 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
         const uint32_t tm = T_::CW_TERM[(p * C_::TW + q) * 64 + lane];
         acc = fma(T_::COEF[tm >> 16], scr[tm & 0xFFFFu], acc);
       }
+      // every lane updates: padding lanes subtract 0 from an entry no other
+      // lane of the pass touches
       const int ad = T_::CW_ADDR[p * 64 + lane];
-      if (ad != 0xFFFF) lu[ad] = lu[ad] - acc;
+      lu[ad] = lu[ad] - acc;
     }
   };
 

@@ -102,6 +102,19 @@ __device__ __forceinline__ double bank(const double (&v)[NB], int row) {
   }
 }
 
+// Zeroes the LU storage lu[0, SPAN] with 16-byte stores (the LDS block starts
+// 16-byte aligned), unrolled: no loop branch per row of 64 lanes.
+template <class W>
+__device__ __forceinline__ void wave_zero_lu(double* lu, int lane) {
+  constexpr int NZ = W::SPAN + 1;
+  double2* lu2 = reinterpret_cast<double2*>(lu);
+#pragma unroll
+  for (int i = 0; i < NZ / 128; ++i) lu2[i * 64 + lane] = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int i = (NZ / 128) * 128; i < NZ; i += 64)
+    if (i + lane < NZ) lu[i + lane] = 0.0;
+}
+
 // F(y) by row slot from the state in ylds (rows, fixed species, ONE), in
 // three steps over the scratch area:
 // 1. lanes take flux groups: scr[g] = k_g * y_a * y_b * y_c, scr[NG] = 0.0;
@@ -267,7 +280,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
       }
     }
     if (lane == 0) scr[W::NPART] = 0.0;
-    for (int i = lane; i <= W::SPAN; i += 64) lu[i] = 0.0;
+    wave_zero_lu<W>(lu, lane);
 #pragma unroll
     for (int b = 0; b < NB; ++b)
       if (valid[b]) lu[rb[b] + rowv[b]] = inv_g_dt;
@@ -279,8 +292,10 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
         const uint32_t tm = T_::CW_TERM[(p * C_::TW + q) * 64 + lane];
         acc = fma(T_::COEF[tm >> 16], scr[tm & 0xFFFFu], acc);
       }
+      // every lane updates: padding lanes subtract 0 from an entry no other
+      // lane of the pass touches
       const int ad = T_::CW_ADDR[p * 64 + lane];
-      if (ad != 0xFFFF) lu[ad] = lu[ad] - acc;
+      lu[ad] = lu[ad] - acc;
     }
   };
 

@@ -634,6 +634,19 @@ def build_code_tables(t: dict, tf: int = F_CHUNK_TERMS, tw: int = 2) -> dict:
 
     ft = build_f_tables(t, cidx, zero_c, tf)
     wterm, waddr = lay(wpasses, tw, ZW, 0xFFFF)
+    # Padding lanes of a W pass subtract 0 from an LU entry that no other lane
+    # of the pass updates (distinct per lane), so every lane's read-modify-write
+    # is unconditional and leaves those entries exactly as they are.  When a
+    # small mechanism runs out of such entries, the rest share the padding slot
+    # SPAN: they all read it and write the same value back.
+    entries_all = sorted(set(t["waddr"]))
+    for p in range(len(wpasses)):
+        row = waddr[p * 64:(p + 1) * 64]
+        spare = iter(a for a in entries_all if a not in set(row))
+        for lane in range(64):
+            if row[lane] == 0xFFFF:
+                waddr[p * 64 + lane] = next(spare, t["span"])
+    assert 0xFFFF not in waddr
     st = build_s_tables(t)
     # adjoint gather nu_g . u: chunks of (row, coefficient) terms per group;
     # padding reads row NV (an always-zero slot) with coefficient 0
