@@ -29,7 +29,7 @@ import numpy as np
 REF_SRC = "/root/reference/src"
 REF_MECH = "/root/reference/mechanisms/openatmos"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fj")
-MECHS = ["chapman", "small_strato", "carbon", "saprc99_mini", "saprc99"]
+MECHS = ["chapman", "small_strato", "carbon", "saprc99_mini", "saprc99", "saprcnov"]
 
 
 def main():

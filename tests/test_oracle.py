@@ -31,7 +31,7 @@ from oracle.oracle import Oracle
 from tests import _mechs
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
-FJ_MECHS = ["chapman", "small_strato", "carbon", "saprc99_mini", "saprc99"]
+FJ_MECHS = ["chapman", "small_strato", "carbon", "saprc99_mini", "saprc99", "saprcnov"]
 
 
 def _plan(name):
@@ -74,7 +74,7 @@ def test_chapman_jacobian_matches_kpp_baseline():
         assert abs(got[k] - v) <= 1e-12 * abs(v), (k, got[k], v)
 
 
-@pytest.mark.parametrize("name", ["chapman", "small_strato", "carbon", "saprc99"])
+@pytest.mark.parametrize("name", ["chapman", "small_strato", "carbon", "saprc99", "saprcnov"])
 def test_jacobian_is_derivative_of_rates(name):
     o = Oracle(name)
     mech = load_mechanism(name)
