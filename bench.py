@@ -28,6 +28,11 @@ PEAK_FP64 = 78.6e12  # MI355X FP64 (vector and matrix) FLOP/s
 HBM_PEAK = 8.0e12
 
 
+def progress(msg):
+    """A line on stderr as each leg ends (stdout carries the one JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -42,6 +47,8 @@ def main():
                     help="skip the differentiation measurement reported as adjoint")
     ap.add_argument("--no-pcie", dest="pcie", action="store_false",
                     help="skip the host-buffer (handle API) run that is reported as pcie_inclusive")
+    ap.add_argument("--no-saprcnov", dest="saprcnov", action="store_false",
+                    help="skip the second wave-engine mechanism reported as saprcnov")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB elementwise copy (x * 1.0) three times: the known-size kernel "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -129,9 +136,15 @@ def main():
         h.close()
         pcie = {"value": ncell / pcie_s, "unit": "grid-cells/s", "ms": pcie_s * 1e3,
                 "what": "mkpp_integrate on host LayoutLeft buffers: host-to-device staging, kernel, copy back"}
+        progress("host-buffer (pcie_inclusive) leg done")
 
+    progress(f"saprc99: {ncell} cells, {float(np.mean(kernel_ms)):.1f} ms per launch")
     batching = run_batching(np) if args.batch else None
+    progress("batching leg done")
     adjoint = run_adjoint(np) if args.adjoint else None
+    progress("adjoint leg done")
+    saprcnov = run_saprcnov(np, ncell, args.cpu_sample // 2, args.cpu_threads) if args.saprcnov else None
+    progress("saprcnov leg done")
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "saprc99_rodas3_262144_traffic.json")
@@ -150,6 +163,7 @@ def main():
                     jvals=np.ascontiguousarray(w["rate_parameters"][:, :m]), atol=w["atol"], rtol=w["rtol"],
                     seed=w["seed"], hoist=False, threads=args.cpu_threads)
         cpu_s = time.perf_counter() - t1
+        progress(f"cpu baseline: {m} cells in {cpu_s:.1f} s")
         cpu = {"value": m / cpu_s, "unit": "grid-cells/s", "cores": args.cpu_threads, "kind": "port",
                "sample": f"first {m} of the {ncell} cells, same inputs, 1 h Rodas3 rtol 1e-3, "
                          f"{cpu_s:.1f} s wall, OpenMP static(64) over cells, rate constants re-evaluated per term"}
@@ -181,10 +195,79 @@ def main():
         "pcie_inclusive": pcie,
         "batching": batching,
         "adjoint": adjoint,
+        "saprcnov": saprcnov,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
     print(json.dumps(out))
+
+
+def run_saprcnov(np, cells, cpu_sample, cpu_threads, reps=2, max_attempts=5000):
+    """The second wave-engine mechanism at the metric's configuration:
+    saprcnov (96 species, 183 flux groups), cells as saprc99_urban with
+    matched photolysis, Rodas3 1 h rtol 1e-3 atol 1e3.  Device-resident
+    throughput over `reps` timed launches (HIP events), its FP64 rate against
+    the peak, and the CPU port on a sample of the same cells.  About 0.25 % of
+    these cells (high NOx, cold) drive the reference controller's step size
+    towards h_min and would run to the 100,000-attempt limit, on the CPU port
+    as on the GPU; both legs stop a cell at `max_attempts` and the count of
+    such cells is reported."""
+    import torch
+
+    from mkpp_amd import native, workloads
+
+    w = workloads.saprcnov_urban(cells)
+    plan = json.load(open(os.path.join(ROOT, "mkpp_amd", "plans", "saprcnov.json")))
+    step_flops = plan["step_flops"]["rodas3"]
+    dev = torch.device("cuda:0")
+    n = w["conc"].shape[0]
+    init = torch.tensor(w["conc"], device=dev).contiguous()
+    rp = torch.tensor(w["rate_parameters"], device=dev).contiguous()
+    T = torch.tensor(w["temp"], device=dev)
+    rho = torch.tensor(w["air_density"], device=dev)
+    at = torch.full((n,), w["atol"], dtype=torch.float64, device=dev)
+    rt = torch.full((n,), w["rtol"], dtype=torch.float64, device=dev)
+    st = torch.zeros(cells, dtype=torch.int32, device=dev)
+    att = torch.zeros(cells, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def launch(buf):
+        native.device_integrate("saprcnov", buf, w["dt"], 1, "rodas3", T, rho, rp, at, rt, w["seed"],
+                                status=st, attempted=att, max_attempts=max_attempts, stream=stream)
+
+    launch(init.clone())
+    bufs = [init.clone() for _ in range(reps)]
+    att.zero_()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for i in range(reps):
+        ev[i][0].record(stream)
+        launch(bufs[i])
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    attempted = att.cpu().numpy().astype(np.float64) / reps
+    achieved = float(attempted.sum()) * step_flops / (ms / 1e3)
+    cpu = None
+    if cpu_sample > 0:
+        from oracle.oracle import Oracle
+
+        m = min(cpu_sample, cells)
+        c = np.ascontiguousarray(w["conc"][:, :m]).copy()
+        t1 = time.perf_counter()
+        Oracle("saprcnov").integrate(c, w["dt"], 1, "rodas3", temp=w["temp"][:m], air_density=w["air_density"][:m],
+                                     jvals=np.ascontiguousarray(w["rate_parameters"][:, :m]), atol=w["atol"],
+                                     rtol=w["rtol"], seed=w["seed"], hoist=False, max_attempts=max_attempts,
+                                     threads=cpu_threads)
+        cpu_s = time.perf_counter() - t1
+        cpu = {"value": m / cpu_s, "unit": "grid-cells/s", "cores": cpu_threads, "kind": "port",
+               "sample": f"first {m} cells, {cpu_s:.1f} s wall"}
+    return {"workload": f"saprcnov Rodas3 1 h rtol 1e-3 atol 1e3, {cells} cells (saprc99 urban setup, "
+                        "matched photolysis, half at night)",
+            "value": cells / (ms / 1e3), "unit": "grid-cells/s", "kernel_ms": ms,
+            "attempted_steps_mean": float(attempted.mean()), "max_attempts": max_attempts,
+            "cells_at_attempt_limit": int((st != 0).sum()),
+            "fp64_tflops": achieved / 1e12, "roofline_frac": achieved / PEAK_FP64, "cpu_baseline": cpu}
 
 
 def run_adjoint(np, cells=65536, max_steps=128):

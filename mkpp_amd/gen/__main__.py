@@ -12,9 +12,9 @@ import os
 import sys
 
 from . import plan as planmod
-from .model import BUNDLED, load_mechanism
+from .model import BATCHED, BUNDLED, load_mechanism
 
-EXTRA = ("saprc99_mini", "saprcnov")
+EXTRA = ("saprc99_mini",)
 
 
 def main(argv=None):
@@ -54,7 +54,7 @@ def main(argv=None):
         registry(names)
         from . import emit_batch
 
-        emit_batch.write([engines[m] for m in names], os.path.join(root, "mkpp_amd", "csrc", "gen"))
+        emit_batch.write([engines[m] for m in names if m in BATCHED], os.path.join(root, "mkpp_amd", "csrc", "gen"))
 
 
 def registry(names):

@@ -50,7 +50,8 @@ def row_displacement(nv: int, pattern: list[list[int]], bank: list[int] | None =
         cols = pattern[i]
         best = None
         for ln in (sorted(free[bank[i]]) if modulus else [None]):
-            r = -min(cols)
+            # R >= 0: the solves read R[row] + k for every column k in every lane
+            r = max(0, -min(cols))
             if modulus:
                 r += (ln - r) % modulus
             while any((r + j) in used for j in cols):

@@ -546,7 +546,10 @@ def load_mechanism(name_or_path: str) -> Mechanism:
     return Mechanism(str(name), data)
 
 
-BUNDLED = ("chapman", "small_strato", "carbon", "saprc99")
+BUNDLED = ("chapman", "small_strato", "carbon", "saprc99", "saprcnov")
+# Mechanisms of the persistent batch kernel (each tableau's kernel carries every
+# one of their integrators).
+BATCHED = ("chapman", "small_strato", "carbon", "saprc99")
 
 # Callable type used by the emitters for k_r code.
 KCode = Callable[[str, str], str]

@@ -13,6 +13,7 @@ tolerances the configuration integrates with.
                     NOx / VOCs rescaled log-uniformly, photolysis scaled by cos(SZA) and
                     half of the cells at night
 * carbon_mixed    - carbon, Rodas3 adaptive, T-dependent rate constants
+* saprcnov_urban  - saprcnov with the saprc99 urban setup and matched photolysis rates
 * batch_mixed     - the batching workload: saprc99 (NOx, VOCs 1e8-1e13, half night),
                     carbon and small_strato (half night) cells, Rodas3 adaptive
                     at rtol 1e-3 for all three in one launch
@@ -91,7 +92,7 @@ SAPRC99_J1 = [6.69e-1, 1.59e0, 1.50e+1, 3.76e-2, 4.19e-3, 1.27e-1, 1.60e-2, 5.40
 
 
 def saprc99_urban(num_cells: int = 262144, seed: int = 3, species=None, nox_range=(1e9, 1e12),
-                  voc_range=(1e9, 1e12), night_fraction=0.5) -> dict:
+                  voc_range=(1e9, 1e12), night_fraction=0.5, jset=None, mechanism="saprc99") -> dict:
     if species is None:
         from .gen.model import load_mechanism
 
@@ -105,12 +106,15 @@ def saprc99_urban(num_cells: int = 262144, seed: int = 3, species=None, nox_rang
     air = number_density(p, T)
     conc = np.zeros((n, num_cells))
     for nm, ppm in SAPRC99_URBAN_PPM.items():
-        conc[idx[nm]] = ppm * cf
+        if nm in idx:  # a species the mechanism lacks is left out
+            conc[idx[nm]] = ppm * cf
     nox = 10.0 ** rng.uniform(math.log10(nox_range[0]), math.log10(nox_range[1]), num_cells)
     conc[idx["NO"]] = nox * 2.0 / 3.0
     conc[idx["NO2"]] = nox / 3.0
     for v in SAPRC99_VOCS:
-        conc[idx[v]] = 10.0 ** rng.uniform(math.log10(voc_range[0]), math.log10(voc_range[1]), num_cells)
+        draw = 10.0 ** rng.uniform(math.log10(voc_range[0]), math.log10(voc_range[1]), num_cells)
+        if v in idx:
+            conc[idx[v]] = draw
     conc[idx["AIR"]] = air
     conc[idx["M"]] = air
     conc[idx["O2"]] = 0.209 * air
@@ -123,8 +127,8 @@ def saprc99_urban(num_cells: int = 262144, seed: int = 3, species=None, nox_rang
     sun = np.cos(np.radians(sza))
     night = rng.uniform(0.0, 1.0, num_cells) < night_fraction
     sun[night] = 0.0
-    jv = np.outer(np.array(SAPRC99_J1) / 60.0, sun)
-    return {"mechanism": "saprc99", "conc": conc, "rate_parameters": jv, "temp": T, "pressure": p,
+    jv = np.outer(np.array(SAPRC99_J1 if jset is None else jset) / 60.0, sun)
+    return {"mechanism": mechanism, "conc": conc, "rate_parameters": jv, "temp": T, "pressure": p,
             "air_density": air, "method": "rodas3", "dt": 3600.0, "steps": 1, "seed": 1.0e-6,
             "atol": 1.0e3, "rtol": 1.0e-3, "night": night}
 
@@ -169,6 +173,37 @@ def batch_mixed(num_saprc99: int = 131072, num_carbon: int = 65536, num_small_st
     for w in (sap, car, sst):
         out.append(dict(w, method="rodas3", dt=3600.0, steps=1, seed=1.0e-6, atol=1.0e3, rtol=1.0e-3))
     return out
+
+
+def saprcnov_jset() -> list:
+    """Photolysis rates (min^-1, overhead sun) for saprcnov's slots: the
+    saprc99 set's rate of the same photolysed species and occurrence (the
+    n-th photolysis of NO2 takes saprc99's n-th NO2 rate), else the median of
+    the saprc99 set."""
+    from .gen.model import load_mechanism
+
+    def keyed(m):
+        seen, out = {}, []
+        for r in m.reactions:
+            if r.kind.upper().startswith("PHOTO"):
+                sp_ = m.names[r.reactants[0][0]]
+                seen[sp_] = seen.get(sp_, 0) + 1
+                out.append((sp_, seen[sp_]))
+        return out
+
+    ref = dict(zip(keyed(load_mechanism("saprc99")), SAPRC99_J1))
+    median = float(np.median(SAPRC99_J1))
+    return [ref.get(key, median) for key in keyed(load_mechanism("saprcnov"))]
+
+
+def saprcnov_urban(num_cells: int = 65536, seed: int = 6, night_fraction=0.5) -> dict:
+    """saprcnov with the saprc99 urban setup (species saprcnov lacks are left
+    out, its extra species start at 0) and saprcnov_jset's photolysis rates;
+    Rodas3 adaptive, rtol 1e-3, atol 1e3, 3600 s."""
+    from .gen.model import load_mechanism
+
+    return saprc99_urban(num_cells, seed=seed, species=load_mechanism("saprcnov").names,
+                         night_fraction=night_fraction, jset=saprcnov_jset(), mechanism="saprcnov")
 
 
 def saprc99_diff(num_cells: int = 65536, seed: int = 5) -> dict:
