@@ -153,7 +153,10 @@ int mkpp_amd_batch_capacity(int method, int* per_cu, int* num_cus);
  * with the same rate law and reactants share one); k arrays are
  * [num_groups x k_stride] per cell, k_stride >= num_cells.  Checkpoints are
  * max_steps * num_variable doubles per cell (ckpt) and max_steps doubles
- * (hs); nsteps[cell] receives the accepted step count.  Device pointers;
+ * (hs); nsteps[cell] receives the accepted step count.  ckpt[(cell *
+ * max_steps + step) * num_variable + row] is the state at the start of that
+ * accepted step, rows in the sparse LU's order (the mechanism's species
+ * permutation, lowering.py:894-996, with the fixed species left out).  Device pointers;
  * enqueued on `stream`; allocates nothing. */
 #define MKPP_CELL_ERR_CHECKPOINT_OVERFLOW 5
 

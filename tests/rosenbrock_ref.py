@@ -102,9 +102,16 @@ class DenseMechanism:
         return (k * np.einsum("cge,cge->cg", d1, de) + dk * pw.prod(axis=2)) @ self.nu.T
 
 
-def replay_tlm(mech: DenseMechanism, y0, k, hs, nsteps, method="rodas3", dy0=None, dk=None):
+def replay_tlm(mech: DenseMechanism, y0, k, hs, nsteps, method="rodas3", dy0=None, dk=None, states=None):
     """Integrate the frozen step sequences (hs[cell, :nsteps[cell]]) from y0 and,
-    if dy0/dk are given, the tangent-linear model along it.  Returns (y, dy)."""
+    if dy0/dk are given, the tangent-linear model along it.  Returns (y, dy).
+
+    states (cells, max_steps, N), optional: the state at the start of every
+    accepted step of another run of the same map (a device checkpoint).  Each
+    step then starts from that state, so the tangent-linear model linearises
+    around that run's trajectory instead of this replay's own, which for an
+    ill-conditioned mechanism drifts away from it over many steps although
+    each step agrees to rounding."""
     tab = TABLEAUX[method]
     S = tab.stages
     g = tab.gamma[0]
@@ -119,6 +126,8 @@ def replay_tlm(mech: DenseMechanism, y0, k, hs, nsteps, method="rodas3", dy0=Non
         if not act.any():
             break
         h = np.where(act, hs[:, n] if n < hs.shape[1] else 1.0, 1.0)
+        if states is not None:
+            y = np.where(act[:, None], states[:, min(n, states.shape[1] - 1)], y)
         J = mech.jacobian(y, k)[:, var][:, :, var]
         W = eye[None] / (g * h)[:, None, None] - J
         K, dK = [], []
