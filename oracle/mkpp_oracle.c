@@ -20,6 +20,10 @@
  *                          N species, controller constants :71-82 and :324-379)
  *   cell loop ............ src/mkpp_host/include/mkpp_host/dispatcher.hpp:224-254
  *                          (per-cell T, air density and forcing row)
+ *   integrate_with_reduction  solver_variants/integrate_with_reduction.j2:51-306
+ *                          (active set from F1, identity rows/columns of W for
+ *                          frozen species, guarded substitutions, its own
+ *                          step controller)
  *
  * Arithmetic follows the reference statement order; it is compiled with
  * -ffp-contract=off like the reference's x86-64 host build.  The rate
@@ -600,6 +604,122 @@ static long integrate_cell(const mech_t *m, const tab_t *tb, double *state, env_
     return attempts;
 }
 
+/* Substitutions of the reduction variant: a frozen row's forward and
+ * backward values are 0.0 (macros/lu_solve_forward.j2:30-31 and
+ * lu_solve_backward.j2:30-31 with is_reduction = true). */
+static void solve_guarded(const mech_t *m, const double *lu, const double *rhs, double *x, double *work, const int *active) {
+    int n = m->n;
+    for (int r = 0; r < n; ++r) {
+        double v = rhs[r];
+        for (int e = m->lo_off[r]; e < m->lo_off[r + 1]; ++e) v -= lu[m->lo_s[e]] * work[m->lo_c[e]];
+        work[r] = active[r] ? v : 0.0;
+    }
+    for (int r = n - 1; r >= 0; --r) {
+        double v = work[r];
+        for (int e = m->up_off[r]; e < m->up_off[r + 1]; ++e) v -= lu[m->up_s[e]] * x[m->up_c[e]];
+        x[r] = active[r] ? v / lu[m->diag[r]] : 0.0;
+    }
+}
+
+/* One cell's integrate_with_reduction(dt_total, importance_threshold)
+ * (integrate_with_reduction.j2:51-306), all indices in the solver's permuted
+ * order as the template writes them:
+ *   - dt starts at min(dt_total, 1.0) (:55);
+ *   - every attempt evaluates F1 at the step's start state and sets
+ *     active[i] = |F1_i| / (atol[i] + rtol[i] |S_i|) >= threshold, with the
+ *     tolerance arrays indexed by the permuted position i (:83-90);
+ *   - W_ii = active_i ? 1/(g dt) - J_ii : 1; W_ij = active_i && active_j ? -J_ij : 0
+ *     (:92-110), factorized by the plain plan (the annotated LU of :112-128 is
+ *     only rendered when annotated expressions exist, and
+ *     lowering.py:1245 annotate_lu_expressions has no caller);
+ *   - stage 1 reuses F1 (:168); substitutions zero frozen rows;
+ *   - error norm over all N with atol[i]/rtol[i] at the permuted position
+ *     (:209-260); accept adds the M sum to active species only (:296);
+ *   - dt *= factor after an accepted and after a rejected step, no h_min,
+ *     no rejection counting (:274-305).
+ * max_attempts > 0 stops the cell after that many attempts, as in
+ * integrate_cell (the template loops without bound). */
+static long integrate_cell_reduced(const mech_t *m, const tab_t *tb, double *state, env_t *e, double dt_total,
+                                   double threshold, const double *atol, const double *rtol, scratch_t *s, int *active,
+                                   long *acc, long max_attempts) {
+    const int n = m->n;
+    const double g = tb->g0, safety = 0.9, max_growth = 6.0, min_shrink = 0.2;
+    double t = 0.0, dt = fmin(dt_total, 1.0);
+    long attempts = 0, accepted = 0;
+    for (int i = 0; i < n; ++i) active[i] = 1;
+    if (e->hoist) env_rates(m, e, state);
+    while (t < dt_total) {
+        if (max_attempts > 0 && attempts >= max_attempts) break;
+        ++attempts;
+        dt = fmin(dt, dt_total - t);
+        const double inv_g_dt = 1.0 / (g * dt);
+        for (int i = 0; i < n; ++i) s->S[i] = state[m->perm[i]];
+        eval_rates(m, e, state, s->fphys);
+        for (int i = 0; i < n; ++i) s->F[i] = s->fphys[m->perm[i]];
+        for (int i = 0; i < n; ++i) active[i] = (fabs(s->F[i]) / (atol[i] + rtol[i] * fabs(state[m->perm[i]])) >= threshold);
+        eval_jacobian(m, e, state, s->J);
+        for (int w = 0; w < m->nw; ++w) {
+            int wi = m->wi[w], wj = m->wj[w];
+            if (wi == wj) {
+                double v = m->wfromj[w] >= 0 ? inv_g_dt - s->J[m->wfromj[w]] : inv_g_dt;
+                s->W[w] = active[wi] ? v : 1.0;
+            } else {
+                s->W[w] = (m->wfromj[w] >= 0 && active[wi] && active[wj]) ? -s->J[m->wfromj[w]] : 0.0;
+            }
+        }
+        oracle_factorize(m, s->W, s->LU);
+        for (int st = 0; st < tb->s; ++st) {
+            int base_a = st * (st - 1) / 2;
+            if (st > 0) {
+                for (int i = 0; i < n; ++i) s->Y[i] = stage_sum(s->S[i], &tb->a[base_a], st, s->K, i, dt, 0);
+                if (tb->newf[st]) {
+                    for (int i = 0; i < n; ++i) s->phys[m->perm[i]] = s->Y[i];
+                    eval_rates(m, e, s->phys, s->fphys);
+                    for (int i = 0; i < n; ++i) s->F[i] = s->fphys[m->perm[i]];
+                }
+                for (int i = 0; i < n; ++i) s->rhs[i] = stage_sum(s->F[i], &tb->c[base_a], st, s->K, i, dt, 1);
+            } else {
+                for (int i = 0; i < n; ++i) s->rhs[i] = s->F[i];
+            }
+            solve_guarded(m, s->LU, s->rhs, s->K[st], s->work, active);
+        }
+        double err_sq = 0.0;
+        for (int i = 0; i < n; ++i) {
+            int pi = m->perm[i], any;
+            double ynew = s->S[i];
+            for (int j = 0; j < tb->s; ++j) {
+                if (tb->mm[j] == 0.0) continue;
+                ynew = ynew + ((tb->mm[j] == 1.0) ? s->K[j][i] : tb->mm[j] * s->K[j][i]);
+            }
+            double ymax = fmax(fabs(state[pi]), fabs(ynew));
+            double sci = atol[i] + rtol[i] * ymax;
+            double yerr = weight_sum(tb->ee, tb->s, s->K, i, &any);
+            err_sq += (yerr / sci) * (yerr / sci);
+        }
+        double err = sqrt(err_sq / n);
+        err = fmax(err, 1.0e-10);
+        double factor;
+        if (tb->elo == 2.0) factor = safety / sqrt(err);
+        else if (tb->elo == 3.0) factor = safety / cbrt(err);
+        else factor = safety * pow(err, -(1.0 / tb->elo));
+        factor = fmax(min_shrink, fmin(factor, max_growth));
+        if (err <= 1.0) {
+            for (int i = 0; i < n; ++i) {
+                int any;
+                double msum = weight_sum(tb->mm, tb->s, s->K, i, &any);
+                if (active[i]) state[m->perm[i]] += any ? msum : 0.0;
+            }
+            t += dt;
+            ++accepted;
+            dt *= factor;
+        } else {
+            dt *= factor;
+        }
+    }
+    *acc = accepted;
+    return attempts;
+}
+
 /* ------------------------------------------------------------------ */
 /* public entry points (ctypes)                                       */
 /* ------------------------------------------------------------------ */
@@ -665,6 +785,56 @@ int oracle_integrate(const mech_t *m, int method, long num_cells, double *conc, 
         }
         free(state);
         free(jv);
+        scratch_free(&s);
+    }
+    return stalled;
+}
+
+/* integrate_with_reduction over num_cells cells (layout as oracle_integrate).
+ * active_out (may be NULL): [num_cells x n] LayoutLeft, the active flags of the
+ * last attempted step in the solver's permuted order. */
+int oracle_integrate_reduced(const mech_t *m, int method, long num_cells, double *conc, const double *temp,
+                             const double *air_density, const double *jvals, double dt_total, int steps,
+                             double threshold, const double *atol, const double *rtol, long *accepted,
+                             long *attempted, long max_attempts, int *active_out, int nthreads) {
+    tab_t tb;
+    get_tableau(method, &tb);
+    int stalled = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel reduction(+ : stalled)
+#endif
+    {
+        scratch_t s;
+        scratch_alloc(m, &s);
+        double *state = dalloc(m->n);
+        double *jv = dalloc(m->nphoto);
+        int *active = ialloc(m->n);
+#ifdef _OPENMP
+#pragma omp for schedule(static, 64)
+#endif
+        for (long c = 0; c < num_cells; ++c) {
+            for (int i = 0; i < m->n; ++i) state[i] = conc[c + num_cells * i];
+            for (int p = 0; p < m->nphoto; ++p) jv[p] = jvals ? jvals[c + num_cells * p] : 0.0;
+            env_t e = {temp ? temp[c] : 300.0, air_density ? air_density[c] : 2.4476e19, jv, 1, s.k, s.dk};
+            long acc_total = 0, att_total = 0;
+            for (int st = 0; st < steps; ++st) {
+                long acc = 0;
+                long att = integrate_cell_reduced(m, &tb, state, &e, dt_total, threshold, atol, rtol, &s, active, &acc,
+                                                  max_attempts);
+                if (max_attempts > 0 && att >= max_attempts) stalled += 1;
+                acc_total += acc;
+                att_total += att;
+            }
+            for (int i = 0; i < m->n; ++i) conc[c + num_cells * i] = state[i];
+            if (active_out)
+                for (int i = 0; i < m->n; ++i) active_out[c + num_cells * i] = active[i];
+            if (accepted) accepted[c] = acc_total;
+            if (attempted) attempted[c] = att_total;
+        }
+        free(state);
+        free(jv);
+        free(active);
         scratch_free(&s);
     }
     return stalled;

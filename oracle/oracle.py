@@ -45,6 +45,9 @@ def lib():
         L.oracle_integrate.restype = C.c_int
         L.oracle_integrate.argtypes = [vp, C.c_int, C.c_long, dp, dp, dp, dp, C.c_double, C.c_int, C.c_double,
                                        dp, dp, lp, lp, C.c_int, C.c_long, C.c_int]
+        L.oracle_integrate_reduced.restype = C.c_int
+        L.oracle_integrate_reduced.argtypes = [vp, C.c_int, C.c_long, dp, dp, dp, dp, C.c_double, C.c_int,
+                                               C.c_double, dp, dp, lp, lp, C.c_long, C.POINTER(C.c_int), C.c_int]
         _lib = L
     return _lib
 
@@ -139,3 +142,30 @@ class Oracle:
                                          int(steps), float(seed), _d(at), _d(rt), _l(acc), _l(att), int(bool(hoist)),
                                          int(max_attempts), int(threads))
         return acc, att, stalled
+
+    def integrate_reduced(self, conc, dt, steps=1, method="rodas3", temp=None, air_density=None, jvals=None,
+                          atol=None, rtol=None, threshold=0.0, max_attempts=0, threads=0):
+        """integrate_with_reduction over cells in place (layout as integrate).
+        atol/rtol are indexed by the solver's permuted position, as the
+        reference's reduction variant indexes them.  Returns (accepted,
+        attempted, stalled_cells, active) with active (num_species, num_cells)
+        the flags of each cell's last attempted step in permuted order."""
+        conc = np.asarray(conc)
+        assert conc.dtype == np.float64 and conc.flags.c_contiguous and conc.shape[0] == self.n
+        ncell = conc.shape[1]
+        T = None if temp is None else np.ascontiguousarray(np.broadcast_to(temp, (ncell,)), dtype=np.float64)
+        rho = None if air_density is None else np.ascontiguousarray(np.broadcast_to(air_density, (ncell,)), dtype=np.float64)
+        jv = None
+        if self.nphoto:
+            jv = np.zeros((self.nphoto, ncell)) if jvals is None else np.ascontiguousarray(jvals, dtype=np.float64)
+            assert jv.shape == (self.nphoto, ncell)
+        at = np.ascontiguousarray(np.broadcast_to(1.0e-3 if atol is None else atol, (self.n,)), dtype=np.float64)
+        rt = np.ascontiguousarray(np.broadcast_to(1.0e-6 if rtol is None else rtol, (self.n,)), dtype=np.float64)
+        acc = np.zeros(ncell, dtype=np.int64)
+        att = np.zeros(ncell, dtype=np.int64)
+        active = np.zeros((self.n, ncell), dtype=np.int32)
+        stalled = lib().oracle_integrate_reduced(self._h, METHODS[method], ncell, _d(conc), _d(T), _d(rho), _d(jv),
+                                                 float(dt), int(steps), float(threshold), _d(at), _d(rt), _l(acc),
+                                                 _l(att), int(max_attempts),
+                                                 active.ctypes.data_as(C.POINTER(C.c_int)), int(threads))
+        return acc, att, stalled, active.astype(bool)
