@@ -101,6 +101,22 @@ int mkpp_amd_device_integrate(const char* mechanism, double* conc, long num_cell
                               double dt, int steps, int method, double initial_step_seed, const double* atol,
                               const double* rtol, int* status, int* accepted, int* attempted, long max_attempts,
                               void* stream);
+/* The reference's integrate_with_reduction
+ * (src/mkpp/templates/solver_variants/integrate_with_reduction.j2:51-306; the
+ * reference compiles it under MKPP_ENABLE_REDUCTION and exposes no C entry
+ * point for it).  Each attempted step freezes the species whose
+ * |F| / (atol[t] + rtol[t] |y|) is below importance_threshold (t: the
+ * species' position in the mechanism's solver permutation, as the reference
+ * indexes the tolerance arrays there; pass uniform tolerances to make the
+ * distinction moot).  Frozen species keep their value for that step.  dt
+ * starts at min(dt, 1 s) and follows the reference's reduced controller (no
+ * h_min, no rejection rules); there is no initial-step seed.  Same layout,
+ * status and attempt limit as mkpp_amd_device_integrate. */
+int mkpp_amd_device_integrate_reduced(const char* mechanism, double* conc, long num_cells, long stride,
+                                      const double* temp, const double* air_density, const double* rate_parameters,
+                                      long parameter_stride, double dt, int steps, int method,
+                                      double importance_threshold, const double* atol, const double* rtol,
+                                      int* status, int* accepted, int* attempted, long max_attempts, void* stream);
 /* what: 0 = rates (out[cell + out_stride*species]),
  *       1 = dense Jacobian (out[cell + out_stride*(i + num_species*j)]). */
 int mkpp_amd_device_evaluate(const char* mechanism, int what, const double* conc, long num_cells, long stride,

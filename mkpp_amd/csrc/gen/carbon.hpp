@@ -30,6 +30,24 @@ struct Mech {
     constexpr int t[] = {-1};
     return t[q];
   }
+  // reduction variant: W locations (LU slot, row, column); tolerance index of a row
+  static constexpr int NWM = 27;
+  static __host__ __device__ constexpr int wm_slot(int e) {
+    constexpr int t[] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26};
+    return t[e];
+  }
+  static __host__ __device__ constexpr int wm_row(int e) {
+    constexpr int t[] = {0, 0, 1, 1, 2, 3, 4, 4, 4, 4, 5, 5, 5, 6, 6, 6, 7, 7, 7, 8, 8, 8, 9, 9, 9, 10, 10};
+    return t[e];
+  }
+  static __host__ __device__ constexpr int wm_col(int e) {
+    constexpr int t[] = {0, 2, 1, 3, 2, 3, 2, 3, 4, 6, 4, 5, 6, 4, 6, 8, 6, 7, 8, 6, 8, 10, 8, 9, 10, 8, 10};
+    return t[e];
+  }
+  static __host__ __device__ constexpr int tol_index(int v) {
+    constexpr int t[] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10};
+    return t[v];
+  }
 
   // k_r and dk_r/d[M] per flux group (src/mkpp/lowering.py:199-313)
   static __device__ __forceinline__ void rate_constants(double T, double M, const double* __restrict__ jv,

@@ -30,6 +30,24 @@ struct Mech {
     constexpr int t[] = {3};
     return t[q];
   }
+  // reduction variant: W locations (LU slot, row, column); tolerance index of a row
+  static constexpr int NWM = 9;
+  static __host__ __device__ constexpr int wm_slot(int e) {
+    constexpr int t[] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+    return t[e];
+  }
+  static __host__ __device__ constexpr int wm_row(int e) {
+    constexpr int t[] = {0, 0, 0, 1, 1, 1, 2, 2, 2};
+    return t[e];
+  }
+  static __host__ __device__ constexpr int wm_col(int e) {
+    constexpr int t[] = {0, 1, 2, 0, 1, 2, 0, 1, 2};
+    return t[e];
+  }
+  static __host__ __device__ constexpr int tol_index(int v) {
+    constexpr int t[] = {1, 2, 3};
+    return t[v];
+  }
 
   // k_r and dk_r/d[M] per flux group (src/mkpp/lowering.py:199-313)
   static __device__ __forceinline__ void rate_constants(double T, double M, const double* __restrict__ jv,

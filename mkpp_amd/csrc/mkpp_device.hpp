@@ -51,6 +51,7 @@ struct DeviceArgs {
   int* accepted;       // [ncells] or null: accepted steps summed over calls
   int* attempted;      // [ncells] or null
   long max_attempts;   // per integrate() call; <= 0 means unbounded
+  double importance_threshold;  // reduction variant only: species below it are frozen
 };
 
 // Differentiation arguments (checkpointed forward, frozen-step replay, adjoint).
@@ -283,6 +284,142 @@ __device__ __forceinline__ int integrate_cell(double (&x)[M::NV], const double (
   return SUCCESS;
 }
 
+// integrate_with_reduction(dt_total, importance_threshold) of one cell (one
+// lane), after src/mkpp/templates/solver_variants/integrate_with_reduction.j2:51-306:
+// every attempt evaluates F1 at the step's start state and keeps species i
+// active when |F1_i| / (atol[t] + rtol[t] |y_i|) >= threshold, t being the
+// species' position in the reference's permutation (the template indexes the
+// tolerance arrays by that position, :86-88 and :237).  A frozen species' row
+// and column of W become the identity's (:92-110) and its stage increments are
+// zero (macros/lu_solve_forward.j2:30-31, lu_solve_backward.j2:30-31).  A
+// frozen row of that W has no off-diagonal entries and a frozen column none
+// either, so the plain factorization leaves them so and a plain solve with the
+// frozen right-hand sides set to zero returns exactly the guarded solve's
+// zeros; the LU and solve code are the plain integrator's.  Controller
+// (:55, :274-305): dt starts at min(dt_total, 1 s) and is multiplied by the
+// clamped factor after every attempt, accepted or not; no h_min.
+template <class M, class Tab>
+__device__ __forceinline__ int integrate_cell_reduced(double (&x)[M::NV], const double (&yf)[M::NF], const double* k,
+                                                      const double* dk, double dt_total, double threshold,
+                                                      const double* __restrict__ atol,
+                                                      const double* __restrict__ rtol, long max_attempts,
+                                                      int& accepted, int& attempted) {
+  constexpr int NV = M::NV, S = Tab::S;
+  const double g = Tab::G;
+  double t = 0.0, dt = fmin(dt_total, 1.0);
+  long attempts = 0;
+  double Sx[NV], K[S][NV], F[NV], LU[M::NLU];
+  bool act[NV];
+  while (t < dt_total) {
+    if (max_attempts > 0 && attempts >= max_attempts) return ERR_CONVERGENCE_FAILURE;
+    ++attempts;
+    ++attempted;
+    dt = fmin(dt, dt_total - t);
+    const double inv_g_dt = 1.0 / (g * dt);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) Sx[i] = x[i];
+    M::rates(Sx, yf, k, F);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int ti = M::tol_index(i);
+      act[i] = fabs(F[i]) / (atol[ti] + rtol[ti] * fabs(Sx[i])) >= threshold;
+    }
+    M::jac_w(Sx, yf, k, dk, inv_g_dt, LU);
+#pragma unroll
+    for (int e = 0; e < M::NWM; ++e) {
+      const int sl = M::wm_slot(e), r = M::wm_row(e), c = M::wm_col(e);
+      LU[sl] = (r == c) ? (act[r] ? LU[sl] : 1.0) : ((act[r] && act[c]) ? LU[sl] : 0.0);
+    }
+    M::factor(LU);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      double rhs[NV];
+      if (s == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) rhs[i] = F[i];
+      } else {
+        const int base = s * (s - 1) / 2;
+        if (Tab::NEWF[s]) {
+          double Y[NV];
+#pragma unroll
+          for (int i = 0; i < NV; ++i) {
+            double v = Sx[i];
+#pragma unroll
+            for (int j = 0; j < s; ++j) {
+              const double a = Tab::A[base + j];
+              if (a == 0.0) continue;
+              v = (a == 1.0) ? v + K[j][i] : v + a * K[j][i];
+            }
+            Y[i] = v;
+          }
+          M::rates(Y, yf, k, F);
+        }
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+          double v = F[i];
+#pragma unroll
+          for (int j = 0; j < s; ++j) {
+            const double c = Tab::C[base + j];
+            if (c == 0.0) continue;
+            v = v + (c / dt) * K[j][i];
+          }
+          rhs[i] = v;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < NV; ++i) rhs[i] = act[i] ? rhs[i] : 0.0;
+      M::solve(LU, rhs, K[s]);
+    }
+    double err_sq = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double ynew = Sx[i];
+      double yerr = 0.0;
+      bool first = true;
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        const double m = Tab::M[j];
+        if (m == 0.0) continue;
+        ynew = ynew + ((m == 1.0) ? K[j][i] : m * K[j][i]);
+      }
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        const double e = Tab::E[j];
+        if (e == 0.0) continue;
+        const double term = (e == 1.0) ? K[j][i] : e * K[j][i];
+        yerr = first ? term : yerr + term;
+        first = false;
+      }
+      const int ti = M::tol_index(i);
+      const double sci = atol[ti] + rtol[ti] * fmax(fabs(x[i]), fabs(ynew));
+      err_sq += (yerr / sci) * (yerr / sci);
+    }
+    double err = sqrt(err_sq / M::N);
+    err = fmax(err, 1.0e-10);
+    const double factor = step_factor<Tab>(err);
+    if (err <= 1.0) {
+#pragma unroll
+      for (int i = 0; i < NV; ++i) {
+        double msum = 0.0;
+        bool first = true;
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+          const double m = Tab::M[j];
+          if (m == 0.0) continue;
+          const double term = (m == 1.0) ? K[j][i] : m * K[j][i];
+          msum = first ? term : msum + term;
+          first = false;
+        }
+        if (act[i]) x[i] += msum;
+      }
+      t += dt;
+      ++accepted;
+    }
+    dt *= factor;
+  }
+  return SUCCESS;
+}
+
 __device__ __forceinline__ bool all_finite(const double* v, int n) {
   bool ok = true;
   for (int i = 0; i < n; ++i) ok = ok && isfinite(v[i]);
@@ -290,8 +427,9 @@ __device__ __forceinline__ bool all_finite(const double* v, int n) {
 }
 
 // One lane per cell: load, hoist rate constants, integrate `steps` times, store.
-// Shared by the plain launch below and the persistent batch kernel.
-template <class M, class Tab>
+// Shared by the plain launch below and the persistent batch kernel; REDUCED
+// selects the reduction variant's integrator.
+template <class M, class Tab, bool REDUCED = false>
 __device__ __forceinline__ void integrate_lane_cell(const DeviceArgs& a, long cell) {
   double x[M::NV], yf[M::NF], jv[M::NPHOTO];
   double k[M::NG], dk[M::NG];
@@ -318,8 +456,13 @@ __device__ __forceinline__ void integrate_lane_cell(const DeviceArgs& a, long ce
   }
   M::rate_constants(T, Mtb, jv, k, dk);
   int acc = 0, att = 0;
-  for (int step = 0; step < a.steps && status == SUCCESS; ++step)
-    status = integrate_cell<M, Tab>(x, yf, k, dk, a.dt, a.seed, a.atol, a.rtol, a.max_attempts, acc, att);
+  for (int step = 0; step < a.steps && status == SUCCESS; ++step) {
+    if constexpr (REDUCED)
+      status = integrate_cell_reduced<M, Tab>(x, yf, k, dk, a.dt, a.importance_threshold, a.atol, a.rtol,
+                                              a.max_attempts, acc, att);
+    else
+      status = integrate_cell<M, Tab>(x, yf, k, dk, a.dt, a.seed, a.atol, a.rtol, a.max_attempts, acc, att);
+  }
 #pragma unroll
   for (int v = 0; v < M::NV; ++v) a.conc[cell + a.stride * M::var_phys(v)] = x[v];
   if (status == SUCCESS && !all_finite(x, M::NV)) status = ERR_NUMERICAL_FAILURE;
@@ -332,6 +475,12 @@ template <class M, class Tab>
 __global__ void __launch_bounds__(128) integrate_lane_kernel(DeviceArgs a) {
   const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (cell < a.ncells) integrate_lane_cell<M, Tab>(a, cell);
+}
+
+template <class M, class Tab>
+__global__ void __launch_bounds__(128) integrate_lane_reduced_kernel(DeviceArgs a) {
+  const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell < a.ncells) integrate_lane_cell<M, Tab, true>(a, cell);
 }
 
 // Evaluation kernels (mkpp_evaluate_rates / mkpp_evaluate_jacobian).

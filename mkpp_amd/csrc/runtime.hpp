@@ -29,6 +29,8 @@ struct MechOps {
   int (*diff)(int mode, const DeviceArgs& a, const AdjArgs& g, hipStream_t s);
   int num_groups;    // flux groups (rate constants)
   int num_variable;  // variable species (checkpoint row length)
+  // integrate_with_reduction (a.importance_threshold; tolerances by permuted position)
+  int (*integrate_reduced)(const DeviceArgs& a, hipStream_t s);
 };
 
 const MechOps* find_mechanism(const char* name);
@@ -47,6 +49,22 @@ int launch_lane_integrate(const DeviceArgs& a, hipStream_t s) {
     case ROS4: integrate_lane_kernel<M, TabRos4><<<grid, block, 0, s>>>(a); break;
     case RODAS3: integrate_lane_kernel<M, TabRodas3><<<grid, block, 0, s>>>(a); break;
     case RODAS4: integrate_lane_kernel<M, TabRodas4><<<grid, block, 0, s>>>(a); break;
+    default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+template <class M>
+int launch_lane_reduced(const DeviceArgs& a, hipStream_t s) {
+  if (a.ncells <= 0) return 0;
+  const dim3 block(128);
+  const dim3 grid((unsigned)((a.ncells + 127) / 128));
+  switch (a.method) {
+    case ROS2: integrate_lane_reduced_kernel<M, TabRos2><<<grid, block, 0, s>>>(a); break;
+    case ROS3: integrate_lane_reduced_kernel<M, TabRos3><<<grid, block, 0, s>>>(a); break;
+    case ROS4: integrate_lane_reduced_kernel<M, TabRos4><<<grid, block, 0, s>>>(a); break;
+    case RODAS3: integrate_lane_reduced_kernel<M, TabRodas3><<<grid, block, 0, s>>>(a); break;
+    case RODAS4: integrate_lane_reduced_kernel<M, TabRodas4><<<grid, block, 0, s>>>(a); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
@@ -88,7 +106,8 @@ int launch_eval_jacobian(const EvalArgs& a, hipStream_t s) {
                                 {F0, F1, F2, F3, F4},                                                      \
                                 nullptr,                                                                   \
                                 M::NG,                                                                     \
-                                M::NV};                                                                    \
+                                M::NV,                                                                     \
+                                &launch_lane_reduced<M>};                                                  \
     return &ops;                                                                                           \
   }                                                                                                        \
   }

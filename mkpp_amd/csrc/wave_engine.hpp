@@ -178,7 +178,16 @@ __device__ __forceinline__ void wave_eval_f(const double* __restrict__ ylds, dou
 // step size of every accepted step (adj->ckpt, adj->hs, adj->nsteps).  MODE 2:
 // replay of recorded step sizes without error control (the frozen-step map
 // the adjoint differentiates).  In modes 1 and 2, adj->k (if set) replaces the
-// rate laws.  The mode branches are compile-time; MODE 0 carries none of them.
+// rate laws.  MODE 3: integrate_with_reduction
+// (src/mkpp/templates/solver_variants/integrate_with_reduction.j2:51-306): F1 is
+// evaluated first and sets the active rows (|F1| / (atol[t] + rtol[t] |y|) >=
+// a.importance_threshold, t the row's position in the reference permutation,
+// as the template indexes the tolerances); after W assembly a pass over the W
+// locations turns frozen rows and columns into the identity's; stage
+// right-hand sides of frozen rows are zero, which with that W makes the plain
+// solves return the template's guarded zeros; dt starts at min(dt, 1 s) and is
+// multiplied by the factor after every attempt.  The mode branches are
+// compile-time; MODE 0 carries none of them.
 template <class W, class Tab, int MODE = 0>
 __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long cell, double* lds, int lane,
                                                     const AdjArgs* adj = nullptr) {
@@ -239,7 +248,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
       kreg[p] = wave_rate(law, &T_::PARAMS[g * 7], Tk, Mtb, jv);
     }
   }
-  if constexpr (MODE != 0) {
+  if constexpr (MODE == 1 || MODE == 2) {
     if (adj->k != nullptr) {
 #pragma unroll
       for (int p = 0; p < NGP; ++p)
@@ -311,7 +320,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
   for (int call = 0; call < a.steps && status == SUCCESS; ++call) {
     const double dt_total = a.dt;
     const double h_min = dt_total * 1.0e-15;
-    double t = 0.0, dt = dt_total * a.seed;
+    double t = 0.0, dt = MODE == 3 ? fmin(dt_total, 1.0) : dt_total * a.seed;
     bool reject_last = false, reject_more = false;
     long attempts = 0;
     int nrep = 0;
@@ -335,10 +344,34 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
       int ln = lane;
       asm volatile("" : "+v"(ln));
       double Sx[NB], K[S][NB], F[NB], rdiag[NB];
+      bool act[NB];
 #pragma unroll
       for (int b = 0; b < NB; ++b) Sx[b] = x[b];
       store_y(Sx);
+      if constexpr (MODE == 3) {
+        // F1 and the active rows; their flags go to facc (free until the next F)
+        eval_rates(F);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const int ti = valid[b] ? (int)T_::TOLIDX[rowv[b]] : 0;
+          act[b] = valid[b] && fabs(F[b]) / (a.atol[ti] + a.rtol[ti] * fabs(Sx[b])) >= a.importance_threshold;
+          if (valid[b]) facc[rowv[b]] = act[b] ? 1.0 : 0.0;
+        }
+      } else {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) act[b] = valid[b];
+      }
       assemble_w(inv_g_dt);
+      if constexpr (MODE == 3) {
+#pragma unroll
+        for (int p = 0; p < W::NMWP; ++p) {
+          const int e = p * 64 + lane;
+          const int ad = T_::MW_ADDR[e], r = T_::MW_ROW[e], c = T_::MW_COL[e];
+          const bool ar = facc[r] != 0.0, ac = facc[c] != 0.0;
+          const double v = lu[ad];
+          lu[ad] = (r == c) ? (ar ? v : 1.0) : ((ar && ac) ? v : 0.0);
+        }
+      }
       C_::factor(lu, rb, ln);
 #pragma unroll
       for (int b = 0; b < NB; ++b) rdiag[b] = valid[b] ? 1.0 / lu[rb[b] + rowv[b]] : 0.0;
@@ -350,7 +383,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
       for (int s = 0; s < S; ++s) {
         double wv[NB];
         const int base = s * (s - 1) / 2;
-        if (s == 0 || Tab::NEWF[s]) {
+        if (s == 0 ? MODE != 3 : Tab::NEWF[s]) {
           if (s > 0) {
             double Y[NB];
 #pragma unroll
@@ -377,7 +410,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
             if (c == 0.0) continue;
             v = v + (c / dt) * K[j][b];
           }
-          wv[b] = valid[b] ? v : 0.0;
+          wv[b] = act[b] ? v : 0.0;
         }
         int ls = ln;
         asm volatile("" : "+v"(ls));
@@ -413,7 +446,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
         }
         // tolerances are read where used (a few cache lines shared by every
         // cell) rather than held in registers through the step
-        const int ph = valid[b] ? (int)T_::VARPHYS[rowv[b]] : 0;
+        const int ph = valid[b] ? (int)(MODE == 3 ? T_::TOLIDX[rowv[b]] : T_::VARPHYS[rowv[b]]) : 0;
         const double sci = a.atol[ph] + a.rtol[ph] * fmax(fabs(x[b]), fabs(ynew));
         term[b] = valid[b] ? (yerr / sci) * (yerr / sci) : 0.0;
       }
@@ -450,15 +483,21 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
             msum = first ? tt : msum + tt;
             first = false;
           }
-          x[b] += msum;
+          if (MODE != 3 || act[b]) x[b] += msum;
         }
         t += dt;
         ++acc_steps;
-        double next_dt = fmax(h_min, fmin(dt * factor_, dt_total));
-        if (reject_last) next_dt = fmin(next_dt, dt);
-        reject_last = false;
-        reject_more = false;
-        dt = next_dt;
+        if constexpr (MODE == 3) {
+          dt *= factor_;
+        } else {
+          double next_dt = fmax(h_min, fmin(dt * factor_, dt_total));
+          if (reject_last) next_dt = fmin(next_dt, dt);
+          reject_last = false;
+          reject_more = false;
+          dt = next_dt;
+        }
+      } else if constexpr (MODE == 3) {
+        dt *= factor_;
       } else {
         dt *= reject_more ? 0.1 : factor_;
         reject_more = reject_last;
@@ -501,6 +540,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   extern __shared__ double lds[];
   const long cell = wave_cell_of_block(base);
   if (cell < a.ncells) wave_integrate_cell<W, Tab>(a, cell, lds, threadIdx.x);
+}
+
+template <class W, class Tab>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) wave_reduced_kernel(DeviceArgs a, long base) {
+  extern __shared__ double lds[];
+  const long cell = wave_cell_of_block(base);
+  if (cell < a.ncells) wave_integrate_cell<W, Tab, 3>(a, cell, lds, threadIdx.x);
 }
 
 // Launch geometry of the one-cell-per-wave kernels: chunks of at most 2^30
@@ -619,6 +665,24 @@ int launch_wave_integrate(const DeviceArgs& a, hipStream_t s) {
 }
 
 template <class W>
+int launch_wave_reduced(const DeviceArgs& a, hipStream_t s) {
+  if (a.ncells <= 0) return 0;
+  const size_t lds = sizeof(double) * W::LDS_DOUBLES;
+  if (a.method < ROS2 || a.method > RODAS4) return -1;
+  for (long base = 0; base < a.ncells; base += WAVE_CHUNK_CELLS) {
+    const unsigned grid = wave_chunk_grid(a.ncells, base);
+    switch (a.method) {
+      case ROS2: wave_reduced_kernel<W, TabRos2><<<grid, 64, lds, s>>>(a, base); break;
+      case ROS3: wave_reduced_kernel<W, TabRos3><<<grid, 64, lds, s>>>(a, base); break;
+      case ROS4: wave_reduced_kernel<W, TabRos4><<<grid, 64, lds, s>>>(a, base); break;
+      case RODAS3: wave_reduced_kernel<W, TabRodas3><<<grid, 64, lds, s>>>(a, base); break;
+      case RODAS4: wave_reduced_kernel<W, TabRodas4><<<grid, 64, lds, s>>>(a, base); break;
+    }
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+template <class W>
 int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
   if (a.ncells <= 0) return 0;
   wave_eval_kernel<W><<<dim3((unsigned)((a.ncells + 63) / 64)), dim3(64), 0, s>>>(a, what);
@@ -684,6 +748,10 @@ int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
     static constexpr const uint16_t* CS_FIX = gen::NAME##_wave::CS_FIX;                                     \
     static constexpr const uint16_t* CG_TERM = gen::NAME##_wave::CG_TERM;                                   \
     static constexpr const uint8_t* CG_GRP = gen::NAME##_wave::CG_GRP;                                      \
+    static constexpr const uint16_t* MW_ADDR = gen::NAME##_wave::MW_ADDR;                                   \
+    static constexpr const uint8_t* MW_ROW = gen::NAME##_wave::MW_ROW;                                      \
+    static constexpr const uint8_t* MW_COL = gen::NAME##_wave::MW_COL;                                      \
+    static constexpr const uint8_t* TOLIDX = gen::NAME##_wave::TOLIDX;                                      \
     using Code = gen::NAME##_wave::Code;                                                                    \
   };                                                                                                        \
   }
@@ -703,7 +771,8 @@ int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
     static const MechOps ops = {#NAME, W::N, W::NUM_REACTIONS, W::NPH, W::NPH, 0,                          \
                                 W::SUPPORTS_RUNTIME_ENVIRONMENT, W::SPECIES,                                \
                                 &launch_wave_integrate<W>, &wave_eval_rates_##NAME, &wave_eval_jac_##NAME, \
-                                {F0, F1, F2, F3, F4}, &launch_wave_diff<W>, W::NG, W::NV};                  \
+                                {F0, F1, F2, F3, F4}, &launch_wave_diff<W>, W::NG, W::NV,                  \
+                                &launch_wave_reduced<W>};                                                   \
     return &ops;                                                                                            \
   }                                                                                                         \
   }

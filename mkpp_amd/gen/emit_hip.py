@@ -240,6 +240,22 @@ def emit(mech: Mechanism, p: dict) -> str:
     w("    constexpr int t[] = {" + (", ".join(str(x) for x in vm.fixed_phys) if nf else "-1") + "};")
     w("    return t[q];")
     w("  }")
+    # reduction variant: W locations (LU slot, row, column) and each row's
+    # position in the reference permutation (its tolerance index there)
+    wset = {tuple(x) for x in vm.w_locs}
+    wm = [(sl, vm.v_of_log[li], vm.v_of_log[lj]) for sl, (li, lj) in enumerate(vm.lu_locs) if (li, lj) in wset]
+    assert all((v, v) in {(r, c) for _, r, c in wm} for v in range(vm.nv)), "a W diagonal is missing"
+    w("  // reduction variant: W locations (LU slot, row, column); tolerance index of a row")
+    w(f"  static constexpr int NWM = {len(wm)};")
+    for fn, col in (("wm_slot", 0), ("wm_row", 1), ("wm_col", 2)):
+        w(f"  static __host__ __device__ constexpr int {fn}(int e) {{")
+        w("    constexpr int t[] = {" + ", ".join(str(x[col]) for x in wm) + "};")
+        w("    return t[e];")
+        w("  }")
+    w("  static __host__ __device__ constexpr int tol_index(int v) {")
+    w("    constexpr int t[] = {" + ", ".join(str(x) for x in vm.var_log) + "};")
+    w("    return t[v];")
+    w("  }")
     w("")
     # rate constants
     w("  // k_r and dk_r/d[M] per flux group (src/mkpp/lowering.py:199-313)")

@@ -288,6 +288,20 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
     for r in range(nv):
         rowof[slot[r]] = r
     waddr = [R[vi] + vj for (vi, vj) in waddr]
+    # reduction variant: every W location of the variable rows in passes of 64
+    # (address, row, column); a short pass repeats its first entry, so every
+    # lane of a pass writes a real entry and duplicates write the same value
+    wm = [(vlog[li], vlog[lj]) for (li, lj) in vm.lu_locs if (li, lj) in wlocs]
+    assert all((v, v) in set(wm) for v in range(nv)), "a W diagonal is missing"
+    nmwp = (len(wm) + 63) // 64
+    mw_addr, mw_row, mw_col = [], [], []
+    for q in range(nmwp):
+        chunk = wm[q * 64:(q + 1) * 64]
+        chunk = chunk + [chunk[0]] * (64 - len(chunk))
+        for (vi, vj) in chunk:
+            mw_addr.append(R[vi] + vj)
+            mw_row.append(vi)
+            mw_col.append(vj)
     lds_doubles = (span + 1) + (nv + nfix + 1) + nscr + (nv + 1)
     return {
         "nscr": nscr, "lds_doubles": lds_doubles,
@@ -300,6 +314,7 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
         "fmask": fmask, "bmask": bmask, "law": law, "photo": photo, "params": params, "gfac": gfac,
         "gnpow": gnpow, "gpow_s": gpow_s, "gpow_p": gpow_p, "poff": poff, "coefs": coefs,
         "fofs": fofs, "fg": fg, "fc": fc, "waddr": waddr, "wdiag": wdiag, "wofs": wofs, "wpi": wpi, "wci": wci,
+        "nmwp": nmwp, "mw_addr": mw_addr, "mw_row": mw_row, "mw_col": mw_col, "tolidx": list(vm.var_log),
         "nentries": len(entries), "num_reactions": p["num_reactions"],
         "supports_runtime_environment": p["supports_runtime_environment"], "species": p["species"],
         "step_flops": p["step_flops"],
@@ -371,12 +386,17 @@ def emit(t: dict) -> str:
         _arr("uint8_t", "GNVC", t["gnvc"]),
         _arr("uint16_t", "SPOFS", t["spofs"]),
         _arr("uint16_t", "SPI", t["spi"]),
+        _arr("uint16_t", "MW_ADDR", t["mw_addr"]),
+        _arr("uint8_t", "MW_ROW", t["mw_row"]),
+        _arr("uint8_t", "MW_COL", t["mw_col"]),
+        _arr("uint8_t", "TOLIDX", t["tolidx"]),
         "",
         "struct Wave {",
         f"  static constexpr const char* NAME = \"{name}\";",
         f"  static constexpr int N = {t['n']}, NV = {t['nv']}, NF = {t['nf']}, NG = {t['ng']};",
         f"  static constexpr int NPH = {t['nph']}, NB = {t['nb']}, SPAN = {t['span']}, NY = {t['ny']};",
         f"  static constexpr int NPART = {t['npart']}, ONE = {t['one']}, NENTRIES = {t['nentries']};",
+        f"  static constexpr int NMWP = {t['nmwp']};  // reduction variant: passes over the W locations",
         f"  static constexpr int THIRD_BODY = {t['third_body']}, NUM_REACTIONS = {t['num_reactions']};",
         f"  static constexpr bool SUPPORTS_RUNTIME_ENVIRONMENT = {'true' if t['supports_runtime_environment'] else 'false'};",
         f"  static constexpr int NSCR = {t['nscr']};  // fluxes or partials + a 0.0 slot; with the row sums: F chunk partials",

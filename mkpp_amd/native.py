@@ -65,6 +65,9 @@ _SIGS = {
                                    C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, C.c_long, _vp], C.c_int),
     "mkpp_amd_device_evaluate": ([C.c_char_p, C.c_int, _vp, C.c_long, C.c_long, _vp, _vp, _vp, C.c_long, _vp, C.c_long,
                                   _vp], C.c_int),
+    "mkpp_amd_device_integrate_reduced": ([C.c_char_p, _vp, C.c_long, C.c_long, _vp, _vp, _vp, C.c_long, C.c_double,
+                                           C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, C.c_long, _vp],
+                                          C.c_int),
 }
 
 
@@ -256,6 +259,27 @@ def device_integrate(mechanism: str, conc, dt: float, steps: int = 1, method: st
         C.c_void_p(s.cuda_stream))
     if rc != MKPP_SUCCESS:
         raise MkppError(rc, "device_integrate")
+
+
+def device_integrate_reduced(mechanism: str, conc, dt: float, steps: int = 1, method: str = "rodas3", temp=None,
+                             air_density=None, rate_parameters=None, atol=None, rtol=None,
+                             importance_threshold: float = 0.0, status=None, accepted=None, attempted=None,
+                             max_attempts: int = 100000, stream=None):
+    """The reference's integrate_with_reduction on device-resident cells, in
+    place (arguments as device_integrate; atol/rtol are indexed by the
+    species' position in the solver permutation, as the reference does)."""
+    import torch
+
+    n, ncell = conc.shape
+    assert conc.is_cuda and conc.dtype == torch.float64 and conc.is_contiguous()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    rc = lib().mkpp_amd_device_integrate_reduced(
+        mechanism.encode(), _ptr(conc), ncell, ncell, _ptr(temp), _ptr(air_density), _ptr(rate_parameters),
+        ncell if rate_parameters is not None else 0, float(dt), int(steps), METHODS[method],
+        float(importance_threshold), _ptr(atol), _ptr(rtol), _ptr(status), _ptr(accepted), _ptr(attempted),
+        int(max_attempts), C.c_void_p(s.cuda_stream))
+    if rc != MKPP_SUCCESS:
+        raise MkppError(rc, "device_integrate_reduced")
 
 
 def device_evaluate(mechanism: str, what: str, conc, temp=None, air_density=None, rate_parameters=None, stream=None):
