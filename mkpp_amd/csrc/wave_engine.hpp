@@ -346,35 +346,40 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
       double Sx[NB], K[S][NB], F[NB], rdiag[NB];
       bool act[NB];
 #pragma unroll
-      for (int b = 0; b < NB; ++b) Sx[b] = x[b];
+      for (int b = 0; b < NB; ++b) {
+        Sx[b] = x[b];
+        act[b] = valid[b];
+      }
       store_y(Sx);
-      if constexpr (MODE == 3) {
-        // F1 and the active rows; their flags go to facc (free until the next F)
-        eval_rates(F);
+      // W, its LU and 1/diag.  MODE 3 runs this inside the stage loop once F1
+      // (the stage-1 F, evaluated at the loop's one F site) has set the active
+      // rows: a second inlined F evaluation ahead of it costs registers.
+      auto factor_w = [&]() {
+        assemble_w(inv_g_dt);
+        if constexpr (MODE == 3) {
+          uint64_t amask[NB];  // active rows, one lane mask per bank (uniform)
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          const int ti = valid[b] ? (int)T_::TOLIDX[rowv[b]] : 0;
-          act[b] = valid[b] && fabs(F[b]) / (a.atol[ti] + a.rtol[ti] * fabs(Sx[b])) >= a.importance_threshold;
-          if (valid[b]) facc[rowv[b]] = act[b] ? 1.0 : 0.0;
+          for (int b = 0; b < NB; ++b) amask[b] = __ballot(act[b]);
+          auto active_slot = [&](int sl) {
+            uint64_t m = amask[0];
+#pragma unroll
+            for (int b = 1; b < NB; ++b) m = (sl >> 6) == b ? amask[b] : m;
+            return ((m >> (sl & 63)) & 1ull) != 0ull;
+          };
+#pragma unroll
+          for (int p = 0; p < W::NMWP; ++p) {
+            const int e = p * 64 + ln;  // opaque lane id: keeps the table loads next to their use
+            const int ad = T_::MW_ADDR[e], r = T_::MW_RSLOT[e], c = T_::MW_CSLOT[e];
+            const bool ar = active_slot(r), ac = active_slot(c);
+            const double v = lu[ad];
+            lu[ad] = (r == c) ? (ar ? v : 1.0) : ((ar && ac) ? v : 0.0);
+          }
         }
-      } else {
+        C_::factor(lu, rb, ln);
 #pragma unroll
-        for (int b = 0; b < NB; ++b) act[b] = valid[b];
-      }
-      assemble_w(inv_g_dt);
-      if constexpr (MODE == 3) {
-#pragma unroll
-        for (int p = 0; p < W::NMWP; ++p) {
-          const int e = p * 64 + lane;
-          const int ad = T_::MW_ADDR[e], r = T_::MW_ROW[e], c = T_::MW_COL[e];
-          const bool ar = facc[r] != 0.0, ac = facc[c] != 0.0;
-          const double v = lu[ad];
-          lu[ad] = (r == c) ? (ar ? v : 1.0) : ((ar && ac) ? v : 0.0);
-        }
-      }
-      C_::factor(lu, rb, ln);
-#pragma unroll
-      for (int b = 0; b < NB; ++b) rdiag[b] = valid[b] ? 1.0 / lu[rb[b] + rowv[b]] : 0.0;
+        for (int b = 0; b < NB; ++b) rdiag[b] = valid[b] ? 1.0 / lu[rb[b] + rowv[b]] : 0.0;
+      };
+      if constexpr (MODE != 3) factor_w();
       // Stage loop kept rolled so that one copy of the F and solve code serves
       // every stage.  Coefficients are read by the (uniform) stage index; a
       // zero coefficient skips its term and a unit one adds K bare, exactly
@@ -383,7 +388,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
       for (int s = 0; s < S; ++s) {
         double wv[NB];
         const int base = s * (s - 1) / 2;
-        if (s == 0 ? MODE != 3 : Tab::NEWF[s]) {
+        if (s == 0 || Tab::NEWF[s]) {
           if (s > 0) {
             double Y[NB];
 #pragma unroll
@@ -400,6 +405,17 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
             store_y(Y);
           }
           eval_rates(F);
+        }
+        if constexpr (MODE == 3) {
+          if (s == 0) {
+            // F1 = F(S) sets the active rows (integrate_with_reduction.j2:83-90)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+              const int ti = valid[b] ? (int)T_::TOLIDX[rowv[b]] : 0;
+              act[b] = valid[b] && fabs(F[b]) / (a.atol[ti] + a.rtol[ti] * fabs(Sx[b])) >= a.importance_threshold;
+            }
+            factor_w();
+          }
         }
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
@@ -749,8 +765,8 @@ int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
     static constexpr const uint16_t* CG_TERM = gen::NAME##_wave::CG_TERM;                                   \
     static constexpr const uint8_t* CG_GRP = gen::NAME##_wave::CG_GRP;                                      \
     static constexpr const uint16_t* MW_ADDR = gen::NAME##_wave::MW_ADDR;                                   \
-    static constexpr const uint8_t* MW_ROW = gen::NAME##_wave::MW_ROW;                                      \
-    static constexpr const uint8_t* MW_COL = gen::NAME##_wave::MW_COL;                                      \
+    static constexpr const uint8_t* MW_RSLOT = gen::NAME##_wave::MW_RSLOT;                                  \
+    static constexpr const uint8_t* MW_CSLOT = gen::NAME##_wave::MW_CSLOT;                                  \
     static constexpr const uint8_t* TOLIDX = gen::NAME##_wave::TOLIDX;                                      \
     using Code = gen::NAME##_wave::Code;                                                                    \
   };                                                                                                        \

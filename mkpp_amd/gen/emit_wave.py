@@ -289,8 +289,10 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
         rowof[slot[r]] = r
     waddr = [R[vi] + vj for (vi, vj) in waddr]
     # reduction variant: every W location of the variable rows in passes of 64
-    # (address, row, column); a short pass repeats its first entry, so every
-    # lane of a pass writes a real entry and duplicates write the same value
+    # (address, lane slot of its row, lane slot of its column: the kernel keeps
+    # the active flags as one 64-bit lane mask per bank); a short pass repeats
+    # its first entry, so every lane of a pass writes a real entry and
+    # duplicates write the same value
     wm = [(vlog[li], vlog[lj]) for (li, lj) in vm.lu_locs if (li, lj) in wlocs]
     assert all((v, v) in set(wm) for v in range(nv)), "a W diagonal is missing"
     nmwp = (len(wm) + 63) // 64
@@ -300,8 +302,8 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
         chunk = chunk + [chunk[0]] * (64 - len(chunk))
         for (vi, vj) in chunk:
             mw_addr.append(R[vi] + vj)
-            mw_row.append(vi)
-            mw_col.append(vj)
+            mw_row.append(slot[vi])
+            mw_col.append(slot[vj])
     lds_doubles = (span + 1) + (nv + nfix + 1) + nscr + (nv + 1)
     return {
         "nscr": nscr, "lds_doubles": lds_doubles,
@@ -387,8 +389,8 @@ def emit(t: dict) -> str:
         _arr("uint16_t", "SPOFS", t["spofs"]),
         _arr("uint16_t", "SPI", t["spi"]),
         _arr("uint16_t", "MW_ADDR", t["mw_addr"]),
-        _arr("uint8_t", "MW_ROW", t["mw_row"]),
-        _arr("uint8_t", "MW_COL", t["mw_col"]),
+        _arr("uint8_t", "MW_RSLOT", t["mw_row"]),
+        _arr("uint8_t", "MW_CSLOT", t["mw_col"]),
         _arr("uint8_t", "TOLIDX", t["tolidx"]),
         "",
         "struct Wave {",

@@ -790,13 +790,13 @@ int oracle_integrate(const mech_t *m, int method, long num_cells, double *conc, 
     return stalled;
 }
 
-/* integrate_with_reduction over num_cells cells (layout as oracle_integrate).
- * active_out (may be NULL): [num_cells x n] LayoutLeft, the active flags of the
+/* integrate_with_reduction over num_cells cells (layout and hoist as
+ * oracle_integrate).  active_out (may be NULL): [num_cells x n] LayoutLeft, the active flags of the
  * last attempted step in the solver's permuted order. */
 int oracle_integrate_reduced(const mech_t *m, int method, long num_cells, double *conc, const double *temp,
                              const double *air_density, const double *jvals, double dt_total, int steps,
                              double threshold, const double *atol, const double *rtol, long *accepted,
-                             long *attempted, long max_attempts, int *active_out, int nthreads) {
+                             long *attempted, int hoist, long max_attempts, int *active_out, int nthreads) {
     tab_t tb;
     get_tableau(method, &tb);
     int stalled = 0;
@@ -816,7 +816,7 @@ int oracle_integrate_reduced(const mech_t *m, int method, long num_cells, double
         for (long c = 0; c < num_cells; ++c) {
             for (int i = 0; i < m->n; ++i) state[i] = conc[c + num_cells * i];
             for (int p = 0; p < m->nphoto; ++p) jv[p] = jvals ? jvals[c + num_cells * p] : 0.0;
-            env_t e = {temp ? temp[c] : 300.0, air_density ? air_density[c] : 2.4476e19, jv, 1, s.k, s.dk};
+            env_t e = {temp ? temp[c] : 300.0, air_density ? air_density[c] : 2.4476e19, jv, hoist, s.k, s.dk};
             long acc_total = 0, att_total = 0;
             for (int st = 0; st < steps; ++st) {
                 long acc = 0;

@@ -47,7 +47,8 @@ def lib():
                                        dp, dp, lp, lp, C.c_int, C.c_long, C.c_int]
         L.oracle_integrate_reduced.restype = C.c_int
         L.oracle_integrate_reduced.argtypes = [vp, C.c_int, C.c_long, dp, dp, dp, dp, C.c_double, C.c_int,
-                                               C.c_double, dp, dp, lp, lp, C.c_long, C.POINTER(C.c_int), C.c_int]
+                                               C.c_double, dp, dp, lp, lp, C.c_int, C.c_long, C.POINTER(C.c_int),
+                                               C.c_int]
         _lib = L
     return _lib
 
@@ -144,7 +145,7 @@ class Oracle:
         return acc, att, stalled
 
     def integrate_reduced(self, conc, dt, steps=1, method="rodas3", temp=None, air_density=None, jvals=None,
-                          atol=None, rtol=None, threshold=0.0, max_attempts=0, threads=0):
+                          atol=None, rtol=None, threshold=0.0, hoist=True, max_attempts=0, threads=0):
         """integrate_with_reduction over cells in place (layout as integrate).
         atol/rtol are indexed by the solver's permuted position, as the
         reference's reduction variant indexes them.  Returns (accepted,
@@ -166,6 +167,6 @@ class Oracle:
         active = np.zeros((self.n, ncell), dtype=np.int32)
         stalled = lib().oracle_integrate_reduced(self._h, METHODS[method], ncell, _d(conc), _d(T), _d(rho), _d(jv),
                                                  float(dt), int(steps), float(threshold), _d(at), _d(rt), _l(acc),
-                                                 _l(att), int(max_attempts),
+                                                 _l(att), int(bool(hoist)), int(max_attempts),
                                                  active.ctypes.data_as(C.POINTER(C.c_int)), int(threads))
         return acc, att, stalled, active.astype(bool)
