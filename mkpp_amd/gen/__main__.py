@@ -4,6 +4,8 @@ Writes, per bundled mechanism:
   mkpp_amd/plans/<mech>.json      plan (groups, slots, LU schedule, flop counts)
   oracle/tables/<mech>.tab        the same plan for the C oracle
   mkpp_amd/csrc/gen/<mech>.hip    HIP/CDNA4 kernels
+  mkpp_amd/artifacts/<mech>_manifest.json, <mech>/factorization_plan.{json,md}
+                                  the reference's on-disk artifacts
 and the persistent batch units over all of them (mkpp_amd/csrc/gen/batch_*).
 """
 
@@ -47,6 +49,9 @@ def main(argv=None):
                 engines[name] = (name, "wave", lds)
         if not args.no_hip and name in BUNDLED:
             compiled.append(name)
+        from . import artifacts
+
+        artifacts.write(p, engines.get(name, (None, None))[1], os.path.join(root, "mkpp_amd", "artifacts"))
         print(f"{name}: N={len(p['species'])} groups={len(p['groups'])} J={len(p['j_entries'])} "
               f"LU={len(p['lu_locations'])} step_flops(rodas3)={p['step_flops']['rodas3']}", file=sys.stderr)
     if compiled:

@@ -11,7 +11,11 @@ Reads the shipped generated headers under
 * ``w_map``: W slot -> [is_diagonal, J slot or -1] (``W_values[w] = ...``);
 * ``operations`` / ``products``: ``factorize_plan`` tables;
 * ``lower`` / ``upper`` / offsets / ``diagonal``: ``solve_plan`` tables;
-* ``num_runtime_parameters``, ``supports_runtime_environment``.
+* ``num_runtime_parameters``, ``supports_runtime_environment``;
+
+and copies each mechanism's shipped ``<mech>/factorization_plan.md`` (a
+generated data file: blocks of the LU operation list and their dependencies)
+to ``factorization_plans/<mech>.md``.
 
 Usage: ``python tests/golden/extract_reference_plans.py`` (build container only).
 """
@@ -97,6 +101,10 @@ def main():
             json.dump(data, f, separators=(",", ":"))
         print(m, len(data["species"]), "J", len(data["j_entries"]), "W", len(data["w_map"]),
               "ops", len(data["operations"]), "prod", len(data["products"]), file=sys.stderr)
+        md_out = os.path.join(os.path.dirname(OUT), "factorization_plans")
+        os.makedirs(md_out, exist_ok=True)
+        with open(os.path.join(REF, m, "factorization_plan.md")) as src, open(os.path.join(md_out, m + ".md"), "w") as dst:
+            dst.write(src.read())
 
 
 if __name__ == "__main__":
