@@ -1,4 +1,4 @@
-"""On-disk artifacts a host build system consumes, in the reference's formats.
+"""On-disk artifacts the reference's generator writes, in its formats.
 
 The reference's generator writes, next to a mechanism's kernels
 (src/mkpp/codegen.py:229-272):
