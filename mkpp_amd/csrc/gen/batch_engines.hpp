@@ -5,7 +5,9 @@
 #include "small_strato.hpp"
 #include "carbon.hpp"
 #include "saprc99_wcode.hpp"
+#include "saprcnov_wcode.hpp"
 MKPP_WAVE_TABLES(saprc99)
+MKPP_WAVE_TABLES(saprcnov)
 
 namespace mkpp {
 struct BatchEngines {
@@ -20,6 +22,7 @@ struct BatchEngines {
       case 1: batch_lane_item<gen::small_strato::Mech, Tab>(a, item, m); break;
       case 2: batch_lane_item<gen::carbon::Mech, Tab>(a, item, m); break;
       case 3: wave_integrate_cell<gen::saprc99_wave::Wave, Tab>(a, item, lds, lane); break;
+      case 4: wave_integrate_cell<gen::saprcnov_wave::Wave, Tab>(a, item, lds, lane); break;
       default: break;
     }
   }

@@ -549,7 +549,7 @@ def load_mechanism(name_or_path: str) -> Mechanism:
 BUNDLED = ("chapman", "small_strato", "carbon", "saprc99", "saprcnov")
 # Mechanisms of the persistent batch kernel (each tableau's kernel carries every
 # one of their integrators).
-BATCHED = ("chapman", "small_strato", "carbon", "saprc99")
+BATCHED = ("chapman", "small_strato", "carbon", "saprc99", "saprcnov")
 
 # Callable type used by the emitters for k_r code.
 KCode = Callable[[str, str], str]

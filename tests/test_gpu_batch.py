@@ -1,5 +1,6 @@
 """The persistent batch kernel (one launch over saprc99, carbon and small_strato
-cells fed from device queues) against one plain launch per mechanism.
+cells fed from device queues, and saprcnov beside them) against one plain
+launch per mechanism.
 
 Both paths run the same per-cell device code, so the results must be
 identical bit for bit, whatever the queue order, grid size or cells per lane.
@@ -68,6 +69,25 @@ def test_batch_matches_plain_launches(mixed):
     _same(got, want)
     for _, st, acc, _ in got:
         assert (st == 0).all() and (acc > 0).all()
+
+
+
+
+def test_batch_with_saprcnov_matches_plain_launches(mixed):
+    """Both wave-engine mechanisms in one launch with the lane mechanisms."""
+    nov = workloads.saprcnov_urban(4096)
+    nov = dict(nov, conc=np.ascontiguousarray(nov["conc"][:, :256]),
+               rate_parameters=np.ascontiguousarray(nov["rate_parameters"][:, :256]),
+               temp=np.ascontiguousarray(nov["temp"][:256]), air_density=np.ascontiguousarray(nov["air_density"][:256]))
+    ws = list(mixed) + [nov]
+    ref = [_device(w) for w in ws]
+    _plain(ref)
+    segs = [_device(w) for w in ws]
+    native.device_integrate_batch(segs, 3600.0, 1, "rodas3")
+    torch.cuda.synchronize()
+    got = _host(segs)
+    _same(got, _host(ref))
+    assert (got[-1][1] == 0).all() and (got[-1][2] > 0).all()
 
 
 @pytest.mark.parametrize("cells_per_lane,workgroups", [(2, 0), (1, 24), (4, 8)])
