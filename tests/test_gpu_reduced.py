@@ -98,6 +98,30 @@ def test_reduced_matches_oracle(mech, threshold):
     assert same.sum() >= 0.9 * done.sum() and r_same <= 1.0 and r_diff <= 10.0
 
 
+@pytest.mark.parametrize("mech", ["saprc99", "carbon"])
+@pytest.mark.parametrize("method", ["ros2", "ros3", "ros4", "rodas4"])
+def test_reduced_every_method_matches_oracle(mech, method):
+    """The other four methods of both engines, at the case's frozen-species threshold."""
+    make, _, dt, atol, rtol, ths = CASES[mech]
+    w = make()
+    w = dict(w, conc=np.ascontiguousarray(w["conc"][:, :128]),
+             rate_parameters=np.ascontiguousarray(w["rate_parameters"][:, :128]),
+             temp=np.ascontiguousarray(w["temp"][:128]), air_density=np.ascontiguousarray(w["air_density"][:128]))
+    ref = w["conc"].copy()
+    racc, ratt, _, _ = Oracle(mech).integrate_reduced(ref, dt, 1, method, temp=w["temp"], air_density=w["air_density"],
+                                                      jvals=w["rate_parameters"], atol=atol, rtol=rtol,
+                                                      threshold=ths[-1], max_attempts=MAXA)
+    got, st, acc, att = _gpu(mech, w, method, dt, atol, rtol, ths[-1])
+    assert (st == 0).all() and (ratt < MAXA).all()
+    r = np.abs(got - ref) / (atol + rtol * np.maximum(np.abs(got), np.abs(ref)))
+    same = acc == racc
+    r_same = r[:, same].max() if same.any() else 0.0
+    r_diff = r[:, ~same].max() if (~same).any() else 0.0
+    print(f"{mech} {method} threshold {ths[-1]}: same step counts {same.mean():.3f}, accepted mean {acc.mean():.1f}, "
+          f"max scaled diff there {r_same:.3e}, elsewhere {r_diff:.3e}")
+    assert same.mean() >= 0.9 and r_same <= 1.0 and r_diff <= 10.0
+
+
 @pytest.mark.parametrize("mech", list(CASES))
 def test_infinite_threshold_freezes_everything(mech):
     make, method, dt, atol, rtol, _ = CASES[mech]
