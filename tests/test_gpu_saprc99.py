@@ -70,14 +70,24 @@ def test_saprc99_rodas3_matches_oracle(urban):
 
 
 def test_saprc99_within_rtol_of_tight_reference(urban):
+    """Against an oracle run at rtol 1e-10: the GPU at the requested
+    tolerances is exactly as accurate as the reference algorithm at them.  The
+    global error of an adaptive run is controlled per step, so it exceeds the
+    local tolerance in a few species (the oracle at rtol 1e-3 reaches 37 units
+    of atol + rtol |y|, with 99 % of the values within 1); per value, the
+    GPU's distance to the tight run exceeds the oracle's own by at most 0.01
+    units, and its 99th percentile is within 1."""
     w = _subset(urban, 96)
     got, st, _, _ = _gpu(w)
     tight, _, _ = _oracle(w, rtol=1e-10, atol=1.0)
+    same, _, _ = _oracle(w)
     assert (st == 0).all()
-    # global error of an adaptive run is a small multiple of the local tolerance
     r = _scaled(got, tight, w["atol"], w["rtol"])
-    print("saprc99 vs rtol 1e-10 reference: max", r.max(), "median", np.median(r))
-    assert np.median(r) <= 1.0 and r.max() <= 50.0
+    r_ref = _scaled(same, tight, w["atol"], w["rtol"])
+    print("saprc99 vs rtol 1e-10 reference: max", r.max(), "median", np.median(r), "p99", np.quantile(r, 0.99),
+          "| the oracle at the same tolerances: max", r_ref.max(), "p99", np.quantile(r_ref, 0.99))
+    assert np.median(r) <= 1.0 and np.quantile(r, 0.99) <= 1.0 and r.max() <= 50.0
+    assert (r <= r_ref + 0.01).all()
 
 
 def test_saprc99_fixed_step_parity(urban):
