@@ -16,7 +16,7 @@ import sys
 from . import plan as planmod
 from .model import BATCHED, BUNDLED, load_mechanism
 
-EXTRA = ("saprc99_mini",)
+EXTRA = ("saprc99_mini", "ts1")
 
 
 def main(argv=None):

@@ -164,7 +164,12 @@ class RateLaw:
     """k_r as a function of temperature and (optionally) the third-body density.
 
     ``kind`` is one of PHOTOLYSIS, ARRHENIUS, TROE, EP2, EP3, HETEROGENEOUS,
-    CONSTANT.  ``uses_third_body`` marks laws whose value depends on [M].
+    CONSTANT, PARAMETER.  ``uses_third_body`` marks laws whose value depends
+    on [M].  PHOTOLYSIS reads runtime parameter ``photo_index`` (the
+    photolysis count in JSON order); PARAMETER reads runtime parameter
+    ``photo_index`` = the reaction's index, the reference's ``Rate_<idx>``
+    for reaction types without an ``A`` (lowering.py:329-335 mapped to
+    ``jvals[idx]`` by format_eqn.py:189-191: USER_DEFINED, SURFACE).
     """
 
     kind: str
@@ -177,7 +182,7 @@ class RateLaw:
 
     @property
     def depends_on_t(self) -> bool:
-        if self.kind in ("PHOTOLYSIS", "HETEROGENEOUS"):
+        if self.kind in ("PHOTOLYSIS", "PARAMETER", "HETEROGENEOUS"):
             return False
         if self.kind == "CONSTANT":
             return self.p["A"].depends_on_t()
@@ -206,7 +211,7 @@ class RateLaw:
     def value_and_dm(self, T: float, m: float, jvals) -> tuple[float, float]:
         """Return (k, dk/d[M])."""
         p = self.p
-        if self.kind == "PHOTOLYSIS":
+        if self.kind in ("PHOTOLYSIS", "PARAMETER"):
             return float(jvals[self.photo_index]), 0.0
         if self.kind == "ARRHENIUS":
             A = p["A"].evaluate(T)
@@ -257,7 +262,7 @@ def _troe_pair(a, b, c):
     return A, B, C
 
 
-def _rate_law(rtype: str, r: dict, photo_index: int) -> RateLaw:
+def _rate_law(rtype: str, r: dict, photo_index: int, reaction_index: int = -1) -> RateLaw:
     t = rtype.upper()
     if t == "PHOTOLYSIS":
         return RateLaw("PHOTOLYSIS", {}, photo_index)
@@ -287,6 +292,8 @@ def _rate_law(rtype: str, r: dict, photo_index: int) -> RateLaw:
         return RateLaw("HETEROGENEOUS", {"gamma": parse_param(r["gamma"])})
     if "A" in r:
         return RateLaw("CONSTANT", {"A": parse_param(r["A"])})
+    if t in ("USER_DEFINED", "SURFACE") and reaction_index >= 0:
+        return RateLaw("PARAMETER", {}, reaction_index)
     raise MechanismError(f"reaction type {rtype} needs an externally supplied rate; not supported")
 
 
@@ -348,7 +355,7 @@ class Mechanism:
 
         self.reactions: list[Reaction] = []
         photo = 0
-        for r in data.get("reactions", []):
+        for ridx, r in enumerate(data.get("reactions", [])):
             rtype = r.get("type", "UNKNOWN")
             if rtype.upper() == "EQUILIBRIUM":
                 raise MechanismError("EQUILIBRIUM blocks are outside this build")
@@ -359,7 +366,7 @@ class Mechanism:
             else:
                 ins = _species_dict(r.get("reactants", {}))
                 outs = _species_dict(r.get("products", {}))
-            law = _rate_law(rtype, r, photo)
+            law = _rate_law(rtype, r, photo, ridx)
             if law.kind == "PHOTOLYSIS":
                 photo += 1
             powers = []
@@ -395,7 +402,10 @@ class Mechanism:
 
     @property
     def num_runtime_parameters(self) -> int:
-        return self.n_photo
+        """Photolysis count, or one past the highest PARAMETER slot when that is
+        larger (template_context.py:521-523)."""
+        slots = [r.law.photo_index for r in self.reactions if r.law.kind == "PARAMETER"]
+        return max([self.n_photo] + [s + 1 for s in slots])
 
     @property
     def has_runtime_environment(self) -> bool:

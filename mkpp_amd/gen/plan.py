@@ -113,12 +113,13 @@ def build(mech: Mechanism) -> dict:
     groups = []
     for g in mech.groups:
         law = g.law
+        kind = "PHOTOLYSIS" if law.kind == "PARAMETER" else law.kind  # both read jvals[photo]
         groups.append(
             {
-                "law": law.kind,
+                "law": kind,
                 "photo": law.photo_index,
-                "params": {k: (law.p[k].value if law.p[k].is_number else law.p[k].text) for k in LAW_PARAMS[law.kind]},
-                "rpn": {k: rpn(law.p[k]) for k in LAW_PARAMS[law.kind]},
+                "params": {k: (law.p[k].value if law.p[k].is_number else law.p[k].text) for k in LAW_PARAMS[kind]},
+                "rpn": {k: rpn(law.p[k]) for k in LAW_PARAMS[kind]},
                 "powers": [[s, p] for s, p in g.powers],
                 "nu": [[s, c] for s, c in sorted(g.nu.items())],
                 "depends": mech.group_depends(g),

@@ -74,7 +74,7 @@ def test_chapman_jacobian_matches_kpp_baseline():
         assert abs(got[k] - v) <= 1e-12 * abs(v), (k, got[k], v)
 
 
-@pytest.mark.parametrize("name", ["chapman", "small_strato", "carbon", "saprc99", "saprcnov"])
+@pytest.mark.parametrize("name", ["chapman", "small_strato", "carbon", "saprc99", "saprcnov", "ts1"])
 def test_jacobian_is_derivative_of_rates(name):
     o = Oracle(name)
     mech = load_mechanism(name)
@@ -120,7 +120,7 @@ def test_chapman_lu_kat():
     assert np.max(np.abs(dense @ x - np.array([1.0, 2, 3, 4]))) <= 1e-12
 
 
-@pytest.mark.parametrize("name", ["chapman", "small_strato", "carbon", "saprc99_mini", "saprc99", "saprcnov"])
+@pytest.mark.parametrize("name", ["chapman", "small_strato", "carbon", "saprc99_mini", "saprc99", "saprcnov", "ts1"])
 def test_factorize_solve_matches_dense(name):
     o = Oracle(name)
     plan = _plan(name)
