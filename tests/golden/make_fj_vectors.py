@@ -29,7 +29,7 @@ import numpy as np
 REF_SRC = "/root/reference/src"
 REF_MECH = "/root/reference/mechanisms/openatmos"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fj")
-MECHS = ["chapman", "small_strato", "carbon", "saprc99_mini", "saprc99", "saprcnov"]
+MECHS = ["chapman", "small_strato", "carbon", "saprc99_mini", "saprc99", "saprcnov", "ts1"]
 
 
 def main():
@@ -40,7 +40,7 @@ def main():
 
     os.makedirs(OUT, exist_ok=True)
     rng = np.random.default_rng(20261203)
-    for name in MECHS:
+    for name in sys.argv[1:] or MECHS:
         mech = load_mechanism(os.path.join(REF_MECH, name, "mechanism.json"))
         built = _evaluate_reaction_fluxes(mech)
         f = built["f_total"]
@@ -65,12 +65,16 @@ def main():
 
         fexpr = [prep(e) for e in f]
         jexpr = [(i, j, prep(J[i, j])) for i in range(n) for j in range(n) if J[i, j] != 0]
+        # runtime parameters: J_<k> photolysis slots and Rate_<idx> slots of reaction
+        # types without an A (format_eqn.py:187-191 maps both to jvals[...])
+        rate_slots = sorted({int(str(sym)[5:]) for e in fexpr for sym in e.free_symbols if str(sym).startswith("Rate_")})
+        nparam = max([nph] + [k + 1 for k in rate_slots])
         cases = []
         for case in range(3):
             y = 10.0 ** rng.uniform(6, 12, n)
             T = float(rng.uniform(240.0, 310.0))
             rho = float(rng.uniform(1e18, 2.5e19))
-            jv = rng.uniform(0.0, 1e-3, nph)
+            jv = rng.uniform(0.0, 1e-3, nparam)
             if "AIR" in species:
                 y[species.index("AIR")] = rho
             if "M" in species:
@@ -81,6 +85,9 @@ def main():
                 subs[sp.Symbol(f"C_{s}")] = sp.Float(float(y[k]), 40)
             for k in range(nph):
                 subs[sp.Symbol(f"J_{k}")] = sp.Float(float(jv[k]), 40)
+            for k in rate_slots:
+                subs[sp.Symbol(f"Rate_{k}", real=True)] = sp.Float(float(jv[k]), 40)
+                subs[sp.Symbol(f"Rate_{k}")] = sp.Float(float(jv[k]), 40)
             F = [float(sp.N(e.subs(subs), 40)) for e in fexpr]
             Jv = [[i, j, float(sp.N(e.subs(subs), 40))] for i, j, e in jexpr]
             cases.append({"T": T, "air_density": rho, "y": [float(v) for v in y], "jvals": [float(v) for v in jv],
