@@ -97,8 +97,13 @@ template <int NB>
 __device__ __forceinline__ double bank(const double (&v)[NB], int row) {
   if constexpr (NB == 1) {
     return v[0];
-  } else {
+  } else if constexpr (NB == 2) {
     return row < 64 ? v[0] : v[1];
+  } else {
+    double r = v[0];
+#pragma unroll
+    for (int b = 1; b < NB; ++b) r = row >= 64 * b ? v[b] : r;
+    return r;
   }
 }
 

@@ -76,8 +76,8 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
     vm = VarMap(mech, p)
     nv, n = vm.nv, vm.n
     nfix = len(vm.fixed_phys)
-    if nv > 128:
-        raise ValueError("wave engine supports up to 128 variable species")
+    if nv > 254:
+        raise ValueError("wave engine supports up to 254 variable species (rows are bytes, 255 marks no row)")
     # ---- variable-logical LU pattern ---------------------------------------
     vlog = vm.v_of_log
     L = sorted((vlog[i], vlog[j]) for (i, j) in vm.L)
@@ -108,7 +108,8 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
     # Row -> (bank, lane) slots.  Lanes are rows; a pivot's update costs one
     # masked block per bank its rows occupy, so the 64 rows that take part in
     # the most updates (the trailing, nearly dense block of the LU) share bank
-    # 0 and the rest go to bank 1.  Within a bank rows keep ascending order.
+    # 0 and the rest fill the next banks, 64 at a time, by the same ranking.
+    # Within a bank rows keep ascending order.
     nb = (nv + 63) // 64
     part = [0] * nv
     for k in range(nv):
@@ -120,13 +121,15 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
     # the trailing block factorized in registers (rows >= k0) must sit in bank 0
     k0 = max(0, nv - REGISTER_BLOCK)
     if nb == 1:
-        bank0 = list(range(nv))
-        bank1 = []
+        banks = [list(range(nv))]
     else:
         forced = list(range(k0, nv))
         ranked = [r for r in sorted(range(nv), key=lambda r: (-part[r], r)) if r < k0]
-        bank0 = sorted(forced + ranked[:64 - len(forced)])
-        bank1 = sorted(ranked[64 - len(forced):])
+        first = 64 - len(forced)
+        banks = [sorted(forced + ranked[:first])]
+        for b in range(1, nb):
+            banks.append(sorted(ranked[first + (b - 1) * 64:first + b * 64]))
+    assert sorted(r for rows in banks for r in rows) == list(range(nv))
     # solve masks per column k and bank: rows below (forward) / above (backward)
     fmask = [[0] * nb for _ in range(nv)]
     bmask = [[0] * nb for _ in range(nv)]
@@ -269,8 +272,9 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
     # Displacements and lanes together: R[row] = lane (mod 32) where the LDS
     # budget of 8 wavefronts per CU allows it, else mod 16, else no constraint.
     bank = [0] * nv
-    for r in bank1:
-        bank[r] = 1
+    for b, rows in enumerate(banks):
+        for r in rows:
+            bank[r] = b
     for modulus in (32, 16, 0):
         R, span, lane_of = row_displacement(nv, pattern, bank, modulus)
         if not modulus or (span + 1) + (nv + nfix + 1) + nscr + (nv + 1) <= LDS_DOUBLES_8_WAVES:
@@ -280,10 +284,9 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
         for r in range(nv):
             slot[r] = 64 * bank[r] + lane_of[r]
     else:
-        for lane, r in enumerate(bank0):
-            slot[r] = lane
-        for lane, r in enumerate(bank1):
-            slot[r] = 64 + lane
+        for b, rows in enumerate(banks):
+            for lane, r in enumerate(rows):
+                slot[r] = 64 * b + lane
     rowof = [255] * (nb * 64)
     for r in range(nv):
         rowof[slot[r]] = r
