@@ -445,7 +445,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
             for (int b = 0; b < NB; ++b) K[j][b] = xs[b];
           }
       }
-      // error norm: rows summed in logical order (fixed species add zeros)
+      // error norm over all rows (fixed species add zeros)
       double term[NB];
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
@@ -472,11 +472,17 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
         term[b] = valid[b] ? (yerr / sci) * (yerr / sci) : 0.0;
       }
       double err_sq = 0.0;
-      if constexpr (MODE != 2)
-        for (int i = 0; i < NV; ++i) {
-          const int sl = T_::SLOT[i];  // rows summed in logical order
-          err_sq += readlane_d(bank<NB>(term, sl), sl & 63);
-        }
+      if constexpr (MODE != 2) {
+        // every row's term summed over the wavefront: each lane adds its
+        // banks, then a butterfly over the 64 lanes; partners add the same
+        // two values, so every lane ends with the same sum
+        double sum = term[0];
+#pragma unroll
+        for (int b = 1; b < NB; ++b) sum += term[b];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) sum += __shfl_xor(sum, off, 64);
+        err_sq = sum;
+      }
       double err = sqrt(err_sq / W::N);
       err = fmax(err, 1.0e-10);
       const double factor_ = step_factor<Tab>(err);
