@@ -71,6 +71,17 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
     rb[b] = valid[b] ? (int)T_::R[row] : 0;
   }
   const int nst = g.nsteps[cell];
+  // A step count outside [0, maxsteps] cannot index the cell's checkpoint
+  // slice: the cell's outputs are NaN and no checkpoint is read.
+  if (nst < 0 || nst > g.maxsteps) {
+    const double bad = __builtin_nan("");
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      if (valid[b]) g.lambda[cell + a.stride * (long)T_::VARPHYS[rowv[b]]] = bad;
+    if (lane < NF) g.lambda[cell + a.stride * (long)T_::FIXPHYS[lane]] = bad;
+    for (int gi = lane; gi < NG; gi += 64) g.kbar[cell + g.kstride * (long)gi] = bad;
+    return;
+  }
 
   // ---- environment, fixed species, rate constants (as the forward pass) ----
   if (lane < NF) ylds[NV + lane] = a.conc[cell + a.stride * (long)T_::FIXPHYS[lane]];

@@ -1,0 +1,5 @@
+// Generated by mkpp_amd.gen.emit_hip: registers decay (one cell per lane).
+#include "decay.hpp"
+#include "../runtime.hpp"
+
+MKPP_LANE_MECHANISM(decay, 30.0, 41.0, 61.0, 49.0, 108.0)

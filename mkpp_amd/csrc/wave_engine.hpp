@@ -329,7 +329,14 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
     bool reject_last = false, reject_more = false;
     long attempts = 0;
     int nrep = 0;
-    if constexpr (MODE == 2) nrep = adj->nsteps[cell];
+    if constexpr (MODE == 2) {
+      // a recorded count outside [0, maxsteps] cannot index the cell's slice
+      nrep = adj->nsteps[cell];
+      if (nrep < 0 || nrep > adj->maxsteps) {
+        status = ERR_CHECKPOINT_OVERFLOW;
+        nrep = 0;
+      }
+    }
     while (MODE == 2 ? acc_steps < nrep : t < dt_total) {
       if (a.max_attempts > 0 && attempts >= a.max_attempts) {
         status = ERR_CONVERGENCE_FAILURE;

@@ -14,7 +14,7 @@ import os
 import sys
 
 from . import plan as planmod
-from .model import BATCHED, BUNDLED, load_mechanism
+from .model import BATCHED, BUNDLED, FORCE_WAVE, load_mechanism
 
 EXTRA = ("saprc99_mini", "ts1")
 
@@ -39,7 +39,7 @@ def main(argv=None):
 
             outdir = os.path.join(root, "mkpp_amd", "csrc", "gen")
             nv = len(p["species"]) - sum(p["fixed"])
-            if nv <= emit_hip.LANE_MAX_VARIABLE_SPECIES:
+            if nv <= emit_hip.LANE_MAX_VARIABLE_SPECIES and name not in FORCE_WAVE:
                 emit_hip.write(mech, p, outdir)
                 engines[name] = (name, "lane", 0)
             else:

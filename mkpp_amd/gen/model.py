@@ -556,7 +556,12 @@ def load_mechanism(name_or_path: str) -> Mechanism:
     return Mechanism(str(name), data)
 
 
-BUNDLED = ("chapman", "small_strato", "carbon", "saprc99", "saprcnov")
+# decay (X -> nothing at a per-cell rate) is the scalar test equation of the
+# step-size controller's known-answer tests; decay_wave is the same mechanism
+# built on the wave-per-cell engine, so that those tests reach both engines.
+BUNDLED = ("chapman", "small_strato", "carbon", "saprc99", "saprcnov", "decay", "decay_wave")
+# Mechanisms built on the wave-per-cell engine whatever their size.
+FORCE_WAVE = ("decay_wave",)
 # Mechanisms of the persistent batch kernel (each tableau's kernel carries every
 # one of their integrators).
 BATCHED = ("chapman", "small_strato", "carbon", "saprc99", "saprcnov")

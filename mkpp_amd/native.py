@@ -238,6 +238,85 @@ def _ptr(t):
     return None if t is None else C.c_void_p(t.data_ptr())
 
 
+_INFO_CACHE: dict = {}
+
+
+def _info(mechanism: str) -> dict:
+    if mechanism not in _INFO_CACHE:
+        _INFO_CACHE[mechanism] = mechanism_info(mechanism)
+    return _INFO_CACHE[mechanism]
+
+
+def validate(mechanism: str, conc, *, temp=None, air_density=None, rate_parameters=None, atol=None, rtol=None,
+             status=None, accepted=None, attempted=None, k=None, ckpt=None, hs=None, nsteps=None, lam=None,
+             out=None, require_tolerances: bool = True) -> int:
+    """Check every tensor an entry point hands to the kernels against the
+    mechanism's metadata before anything is launched: the kernels address
+    conc[cell + cells*species] for every species, rate parameters for every
+    slot, per-cell arrays for every cell, so a short or mistyped tensor would
+    be read or written out of bounds.  Raises ValueError; returns the cell
+    count.  Shapes and dtypes are checked first, then that every tensor is a
+    contiguous CUDA tensor on conc's device."""
+    import torch
+
+    info = _info(mechanism)
+    N, P = info["num_species"], info["num_rate_parameters"]
+    f64, i32 = torch.float64, torch.int32
+
+    def need(t, name, shape, dtype):
+        if tuple(t.shape) != tuple(shape):
+            raise ValueError(f"{mechanism}: {name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+        if t.dtype != dtype:
+            raise ValueError(f"{mechanism}: {name} has dtype {t.dtype}, expected {dtype}")
+
+    if conc.dim() != 2 or conc.shape[0] != N:
+        raise ValueError(f"{mechanism}: conc has shape {tuple(conc.shape)}, expected ({N}, num_cells)")
+    ncell = int(conc.shape[1])
+    need(conc, "conc", (N, ncell), f64)
+    checks = [("conc", conc)]
+    for name, t, shape, dtype in (("temp", temp, (ncell,), f64), ("air_density", air_density, (ncell,), f64),
+                                  ("atol", atol, (N,), f64), ("rtol", rtol, (N,), f64),
+                                  ("status", status, (ncell,), i32), ("accepted", accepted, (ncell,), i32),
+                                  ("attempted", attempted, (ncell,), i32), ("lam", lam, (N, ncell), f64)):
+        if t is not None:
+            need(t, name, shape, dtype)
+            checks.append((name, t))
+    if require_tolerances and (atol is None or rtol is None):
+        raise ValueError(f"{mechanism}: atol and rtol ({N},) float64 tensors are required")
+    if rate_parameters is not None:
+        if P == 0:
+            raise ValueError(f"{mechanism}: takes no rate parameters")
+        need(rate_parameters, "rate_parameters", (P, ncell), f64)
+        checks.append(("rate_parameters", rate_parameters))
+    if k is not None or ckpt is not None or hs is not None or nsteps is not None:
+        ng, nv = diff_info(mechanism)
+        if k is not None:
+            need(k, "k", (ng, ncell), f64)
+            checks.append(("k", k))
+        if hs is not None:
+            if hs.dim() != 2 or hs.shape[0] != ncell or hs.shape[1] < 1:
+                raise ValueError(f"{mechanism}: hs has shape {tuple(hs.shape)}, expected ({ncell}, max_steps)")
+            need(hs, "hs", (ncell, hs.shape[1]), f64)
+            checks.append(("hs", hs))
+            if ckpt is not None:
+                need(ckpt, "ckpt", (ncell, hs.shape[1], nv), f64)
+                checks.append(("ckpt", ckpt))
+        elif ckpt is not None:
+            raise ValueError(f"{mechanism}: ckpt needs hs")
+        if nsteps is not None:
+            need(nsteps, "nsteps", (ncell,), i32)
+            checks.append(("nsteps", nsteps))
+    if out is not None:
+        checks.append(("out", out))
+    dev = conc.device
+    for name, t in checks:
+        if not t.is_cuda or t.device != dev:
+            raise ValueError(f"{mechanism}: {name} must be a CUDA tensor on {dev}, got {t.device}")
+        if not t.is_contiguous():
+            raise ValueError(f"{mechanism}: {name} must be contiguous")
+    return ncell
+
+
 def device_integrate(mechanism: str, conc, dt: float, steps: int = 1, method: str = "rodas3", temp=None,
                      air_density=None, rate_parameters=None, atol=None, rtol=None, seed: float = 1.0e-6,
                      status=None, accepted=None, attempted=None, max_attempts: int = 100000, stream=None):
@@ -249,8 +328,8 @@ def device_integrate(mechanism: str, conc, dt: float, steps: int = 1, method: st
     """
     import torch
 
-    n, ncell = conc.shape
-    assert conc.is_cuda and conc.dtype == torch.float64 and conc.is_contiguous()
+    ncell = validate(mechanism, conc, temp=temp, air_density=air_density, rate_parameters=rate_parameters,
+                     atol=atol, rtol=rtol, status=status, accepted=accepted, attempted=attempted)
     s = stream if stream is not None else torch.cuda.current_stream()
     rc = lib().mkpp_amd_device_integrate(
         mechanism.encode(), _ptr(conc), ncell, ncell, _ptr(temp), _ptr(air_density), _ptr(rate_parameters),
@@ -270,8 +349,8 @@ def device_integrate_reduced(mechanism: str, conc, dt: float, steps: int = 1, me
     species' position in the solver permutation, as the reference does)."""
     import torch
 
-    n, ncell = conc.shape
-    assert conc.is_cuda and conc.dtype == torch.float64 and conc.is_contiguous()
+    ncell = validate(mechanism, conc, temp=temp, air_density=air_density, rate_parameters=rate_parameters,
+                     atol=atol, rtol=rtol, status=status, accepted=accepted, attempted=attempted)
     s = stream if stream is not None else torch.cuda.current_stream()
     rc = lib().mkpp_amd_device_integrate_reduced(
         mechanism.encode(), _ptr(conc), ncell, ncell, _ptr(temp), _ptr(air_density), _ptr(rate_parameters),
@@ -285,7 +364,11 @@ def device_integrate_reduced(mechanism: str, conc, dt: float, steps: int = 1, me
 def device_evaluate(mechanism: str, what: str, conc, temp=None, air_density=None, rate_parameters=None, stream=None):
     import torch
 
-    n, ncell = conc.shape
+    if what not in ("rates", "jacobian"):
+        raise ValueError(f"what must be 'rates' or 'jacobian', got {what!r}")
+    ncell = validate(mechanism, conc, temp=temp, air_density=air_density, rate_parameters=rate_parameters,
+                     require_tolerances=False)
+    n = conc.shape[0]
     if what == "rates":
         out = torch.empty((n, ncell), dtype=torch.float64, device=conc.device)
     else:
@@ -326,8 +409,9 @@ def device_integrate_batch(segments, dt: float, steps: int = 1, method: str = "r
     arr = (BatchSegment * len(segments))()
     for i, g in enumerate(segments):
         conc = g["conc"]
-        assert conc.is_cuda and conc.dtype == torch.float64 and conc.is_contiguous()
-        n, ncell = conc.shape
+        ncell = validate(g["mechanism"], conc, temp=g.get("temp"), air_density=g.get("air_density"),
+                         rate_parameters=g.get("rate_parameters"), atol=g.get("atol"), rtol=g.get("rtol"),
+                         status=g.get("status"), accepted=g.get("accepted"), attempted=g.get("attempted"))
         name = g["mechanism"].encode()
         keep.append(name)
         rp = g.get("rate_parameters")
@@ -369,7 +453,8 @@ def device_rate_constants(mechanism: str, conc, temp=None, air_density=None, rat
     """Rate constants k (groups, cells) from the rate laws (device tensors)."""
     import torch
 
-    n, ncell = conc.shape
+    ncell = validate(mechanism, conc, temp=temp, air_density=air_density, rate_parameters=rate_parameters,
+                     require_tolerances=False)
     ng, _ = diff_info(mechanism)
     k = torch.empty((ng, ncell), dtype=torch.float64, device=conc.device)
     rc = lib().mkpp_amd_device_rate_constants(mechanism.encode(), _ptr(conc), ncell, ncell, _ptr(temp),
@@ -387,9 +472,12 @@ def device_integrate_checkpointed(mechanism: str, conc, dt: float, method: str, 
     """Forward integration of device-resident cells that records (replay=False)
     or replays (replay=True) the accepted steps.  ckpt: (cells, max_steps, NV)
     float64, hs: (cells, max_steps) float64, nsteps: (cells,) int32."""
-    n, ncell = conc.shape
+    if ckpt is None or hs is None or nsteps is None:
+        raise ValueError(f"{mechanism}: ckpt, hs and nsteps are required")
+    ncell = validate(mechanism, conc, temp=temp, air_density=air_density, rate_parameters=rate_parameters,
+                     atol=atol, rtol=rtol, status=status, accepted=accepted, attempted=attempted, k=k, ckpt=ckpt,
+                     hs=hs, nsteps=nsteps)
     max_steps = hs.shape[1]
-    assert conc.is_contiguous() and ckpt.is_contiguous() and hs.is_contiguous() and ckpt.shape[:2] == hs.shape
     rc = lib().mkpp_amd_device_integrate_checkpointed(
         mechanism.encode(), _ptr(conc), ncell, ncell, _ptr(temp), _ptr(air_density), _ptr(rate_parameters),
         ncell if rate_parameters is not None else 0, _ptr(k), ncell if k is not None else 0, float(dt),
@@ -405,10 +493,12 @@ def device_adjoint(mechanism: str, conc, method: str, temp, air_density, rate_pa
     dL/dy(0) out (in place).  Returns kbar (groups, cells) = dL/dk."""
     import torch
 
-    n, ncell = conc.shape
+    if ckpt is None or hs is None or nsteps is None or lam is None:
+        raise ValueError(f"{mechanism}: ckpt, hs, nsteps and lam are required")
+    ncell = validate(mechanism, conc, temp=temp, air_density=air_density, rate_parameters=rate_parameters, k=k,
+                     ckpt=ckpt, hs=hs, nsteps=nsteps, lam=lam, require_tolerances=False)
     ng, _ = diff_info(mechanism)
     kbar = torch.empty((ng, ncell), dtype=torch.float64, device=conc.device)
-    assert lam.is_contiguous() and lam.shape == conc.shape
     rc = lib().mkpp_amd_device_adjoint(
         mechanism.encode(), _ptr(conc), ncell, ncell, _ptr(temp), _ptr(air_density), _ptr(rate_parameters),
         ncell if rate_parameters is not None else 0, _ptr(k), ncell, METHODS[method], _ptr(ckpt), _ptr(hs),

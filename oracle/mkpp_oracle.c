@@ -517,9 +517,12 @@ static void scratch_free(scratch_t *s) {
 }
 
 /* One cell's integrate(dt_total): state in physical order, updated in place.
- * Returns the number of attempted steps (accepted in *acc). */
+ * Returns the number of attempted steps (accepted in *acc).  trace (may be
+ * NULL) receives, per attempt while fewer than trace_cap, the four values
+ * {t at the attempt, its dt, its error norm, 1 if accepted else 0}. */
 static long integrate_cell(const mech_t *m, const tab_t *tb, double *state, env_t *e, double dt_total, double seed,
-                           const double *atol, const double *rtol, scratch_t *s, long *acc, long max_attempts) {
+                           const double *atol, const double *rtol, scratch_t *s, long *acc, long max_attempts,
+                           double *trace, long trace_cap) {
     const int n = m->n;
     const double g = tb->g0, safety = 0.9, max_growth = 6.0, min_shrink = 0.2, rej_dec = 0.1;
     const double h_min = dt_total * 1.0e-15;
@@ -581,6 +584,10 @@ static long integrate_cell(const mech_t *m, const tab_t *tb, double *state, env_
         else if (tb->elo == 3.0) factor = safety / cbrt(err);
         else factor = safety * pow(err, -(1.0 / tb->elo));
         factor = fmax(min_shrink, fmin(factor, max_growth));
+        if (trace && attempts <= trace_cap) {
+            double *r = trace + 4 * (attempts - 1);
+            r[0] = t; r[1] = dt; r[2] = err; r[3] = err <= 1.0 ? 1.0 : 0.0;
+        }
         if (err <= 1.0) {
             for (int i = 0; i < n; ++i) {
                 int any;
@@ -774,7 +781,7 @@ int oracle_integrate(const mech_t *m, int method, long num_cells, double *conc, 
             long acc_total = 0, att_total = 0;
             for (int st = 0; st < steps; ++st) {
                 long acc = 0;
-                long att = integrate_cell(m, &tb, state, &e, dt_total, seed, atol, rtol, &s, &acc, max_attempts);
+                long att = integrate_cell(m, &tb, state, &e, dt_total, seed, atol, rtol, &s, &acc, max_attempts, NULL, 0);
                 if (max_attempts > 0 && att >= max_attempts) stalled += 1;
                 acc_total += acc;
                 att_total += att;
@@ -788,6 +795,41 @@ int oracle_integrate(const mech_t *m, int method, long num_cells, double *conc, 
         scratch_free(&s);
     }
     return stalled;
+}
+
+/* One cell's integrate(dt_total) with its attempt trace (integrate_cell's
+ * trace layout, 4 doubles per attempt, at most trace_cap attempts recorded).
+ * state: physical order, updated in place; jv: the cell's rate parameters.
+ * Returns the number of attempted steps; *accepted the accepted ones. */
+long oracle_integrate_trace(const mech_t *m, int method, double *state, double temp, double air_density,
+                            const double *jv, double dt_total, double seed, const double *atol, const double *rtol,
+                            long max_attempts, double *trace, long trace_cap, long *accepted) {
+    tab_t tb;
+    get_tableau(method, &tb);
+    scratch_t s;
+    scratch_alloc(m, &s);
+    env_t e = {temp, air_density, jv, 1, s.k, s.dk};
+    long acc = 0;
+    long att = integrate_cell(m, &tb, state, &e, dt_total, seed, atol, rtol, &s, &acc, max_attempts, trace, trace_cap);
+    scratch_free(&s);
+    if (accepted) *accepted = acc;
+    return att;
+}
+
+/* The tableau the oracle integrates with, for comparison against the
+ * reference's published coefficients: out[0] stages, out[1] gamma_1,
+ * out[2] ELO, out[3..17] A, out[18..32] C, out[33..38] M, out[39..44] E,
+ * out[45..50] NewF (1/0); unused entries are 0.  Returns 0, or -1 for an
+ * unknown method. */
+int oracle_tableau(int method, double *out) {
+    if (method < 0 || method > 4) return -1;
+    tab_t tb;
+    get_tableau(method, &tb);
+    memset(out, 0, 51 * sizeof(double));
+    out[0] = tb.s; out[1] = tb.g0; out[2] = tb.elo;
+    for (int i = 0; i < 15; ++i) { out[3 + i] = tb.a[i]; out[18 + i] = tb.c[i]; }
+    for (int i = 0; i < 6; ++i) { out[33 + i] = tb.mm[i]; out[39 + i] = tb.ee[i]; out[45 + i] = tb.newf[i]; }
+    return 0;
 }
 
 /* integrate_with_reduction over num_cells cells (layout and hoist as

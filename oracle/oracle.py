@@ -45,6 +45,11 @@ def lib():
         L.oracle_integrate.restype = C.c_int
         L.oracle_integrate.argtypes = [vp, C.c_int, C.c_long, dp, dp, dp, dp, C.c_double, C.c_int, C.c_double,
                                        dp, dp, lp, lp, C.c_int, C.c_long, C.c_int]
+        L.oracle_integrate_trace.restype = C.c_long
+        L.oracle_integrate_trace.argtypes = [vp, C.c_int, dp, C.c_double, C.c_double, dp, C.c_double, C.c_double,
+                                             dp, dp, C.c_long, dp, C.c_long, lp]
+        L.oracle_tableau.restype = C.c_int
+        L.oracle_tableau.argtypes = [C.c_int, dp]
         L.oracle_integrate_reduced.restype = C.c_int
         L.oracle_integrate_reduced.argtypes = [vp, C.c_int, C.c_long, dp, dp, dp, dp, C.c_double, C.c_int,
                                                C.c_double, dp, dp, lp, lp, C.c_int, C.c_long, C.POINTER(C.c_int),
@@ -144,6 +149,24 @@ class Oracle:
                                          int(max_attempts), int(threads))
         return acc, att, stalled
 
+    def integrate_trace(self, y, dt, method="rodas3", temp=300.0, air_density=2.4476e19, jvals=(), atol=1.0e-3,
+                        rtol=1.0e-6, seed=1.0e-6, max_attempts=0, cap=100000):
+        """One cell's integrate(dt) with its attempt trace.  y: physical-order
+        state (not modified).  Returns (y_end, trace) with trace an
+        (attempts, 4) array of [t, dt, error norm, accepted] per attempt."""
+        y = np.array(y, dtype=np.float64).reshape(-1).copy()
+        assert y.shape[0] == self.n
+        jv = np.ascontiguousarray(np.asarray(jvals, dtype=np.float64).reshape(-1) if len(jvals) else np.zeros(1))
+        at = np.ascontiguousarray(np.broadcast_to(atol, (self.n,)), dtype=np.float64)
+        rt = np.ascontiguousarray(np.broadcast_to(rtol, (self.n,)), dtype=np.float64)
+        tr = np.zeros((cap, 4))
+        acc = np.zeros(1, dtype=np.int64)
+        att = lib().oracle_integrate_trace(self._h, METHODS[method], _d(y), float(temp), float(air_density), _d(jv),
+                                           float(dt), float(seed), _d(at), _d(rt), int(max_attempts), _d(tr), int(cap),
+                                           _l(acc))
+        assert att <= cap, "trace capacity exceeded"
+        return y, tr[:att].copy()
+
     def integrate_reduced(self, conc, dt, steps=1, method="rodas3", temp=None, air_density=None, jvals=None,
                           atol=None, rtol=None, threshold=0.0, hoist=True, max_attempts=0, threads=0):
         """integrate_with_reduction over cells in place (layout as integrate).
@@ -170,3 +193,13 @@ class Oracle:
                                                  _l(att), int(bool(hoist)), int(max_attempts),
                                                  active.ctypes.data_as(C.POINTER(C.c_int)), int(threads))
         return acc, att, stalled, active.astype(bool)
+
+
+def tableau(method: str) -> dict:
+    """The coefficients the oracle integrates with (oracle_tableau)."""
+    out = np.zeros(51)
+    assert lib().oracle_tableau(METHODS[method], _d(out)) == 0
+    s = int(out[0])
+    ntri = s * (s - 1) // 2
+    return {"stages": s, "gamma1": out[1], "elo": out[2], "A": list(out[3:3 + ntri]), "C": list(out[18:18 + ntri]),
+            "M": list(out[33:33 + s]), "E": list(out[39:39 + s]), "NewF": [bool(v) for v in out[45:45 + s]]}
