@@ -7,11 +7,20 @@ oracle's controller (tests/test_integrate_pins.py) go through the C ABI on
 wave-per-cell engine), one cell per run and several runs per launch:
 
 * accepted and attempted step counts equal the closed-form controller's;
-* the final state equals the closed-form run's to 1e-11 relative;
+* the final state equals the closed-form run's to 1e-11 relative on the lane
+  engine (unfused arithmetic in the oracle's order) and 1e-9 on the wave
+  engine;
 * on the wave engine the checkpointed forward records every accepted step's
-  dt, which must equal the closed-form sequence (1e-9 relative), including the
-  x0.1 shrink after repeated rejections, the no-growth step after a rejection,
-  the h_min clamp and the final clipped step.
+  dt, which must equal the closed-form sequence, including the x0.1 shrink
+  after repeated rejections, the no-growth step after a rejection, the h_min
+  clamp and the final clipped step.
+
+The wave engine fuses multiply-adds, and its error estimate sum_s E_s K_s
+cancels its O(lam h) stage values down to O((lam h)^ELO): a last-ulp
+difference in a stage value becomes a relative difference of about
+1e-16 / (lam h)^(ELO-1) in the error norm, and that reaches the next dt
+through err^(-1/ELO).  With lam h down to 4e-3 (Ros4) that is 1e-9 to 1e-8, so
+the wave engine's dt sequence is held to 1e-7 relative.
 """
 
 import numpy as np
@@ -72,7 +81,7 @@ def test_controller_kat_counts_and_state(mech, method):
         assert (st == 0).all()
         assert (att == tr.shape[0]).all(), (run, att[0], tr.shape[0])
         assert (acc == int(tr[:, 3].sum())).all()
-        np.testing.assert_allclose(y, y_exp, rtol=1e-11, atol=0.0)
+        np.testing.assert_allclose(y, y_exp, rtol=1e-11 if mech == "decay" else 1e-9, atol=0.0)
         assert (y == y[0]).all()
 
 
@@ -100,10 +109,11 @@ def test_controller_kat_accepted_dt_sequence_wave(method):
         ns_h = ns.cpu().numpy()
         assert (ns_h == len(acc_dt)).all()
         got = hs.cpu().numpy()[:, :len(acc_dt)]
-        # every step to 1e-9; the final one is dt_total - t, whose cancellation
-        # turns t's rounding (1e-12 relative) into an absolute 1e-11 dt_total
-        np.testing.assert_allclose(got[:, :-1], np.broadcast_to(acc_dt[:-1], got[:, :-1].shape), rtol=1e-9, atol=0.0)
-        np.testing.assert_allclose(got[:, -1], acc_dt[-1], rtol=0.0, atol=1e-11 * T)
+        # every step to 1e-7 (see the module docstring); the final one is
+        # dt_total - t, whose cancellation turns t's rounding into an absolute
+        # 1e-10 dt_total
+        np.testing.assert_allclose(got[:, :-1], np.broadcast_to(acc_dt[:-1], got[:, :-1].shape), rtol=1e-7, atol=0.0)
+        np.testing.assert_allclose(got[:, -1], acc_dt[-1], rtol=0.0, atol=1e-10 * T)
         # checkpoints hold the state at the start of each accepted step
         y_starts = [y0]
         y = y0
@@ -111,10 +121,10 @@ def test_controller_kat_accepted_dt_sequence_wave(method):
             y = sr.attempt(REF[method], lam, h, y)[0]
             y_starts.append(y)
         np.testing.assert_allclose(ck.cpu().numpy()[0, :len(acc_dt), 0], y_starts, rtol=1e-9, atol=0.0)
-        np.testing.assert_allclose(conc.cpu().numpy()[0], y_exp, rtol=1e-11, atol=0.0)
+        np.testing.assert_allclose(conc.cpu().numpy()[0], y_exp, rtol=1e-9, atol=0.0)
         if seed == 1e-17:
             assert got[0, 0] == T * 1e-17 and got[0, 1] == T * 1e-15  # h_min clamp
-        assert got[0, -1] < got[0, -2] * 6.0 and got[0, -1] == pytest.approx(acc_dt[-1], abs=1e-11 * T)  # clipped
+        assert got[0, -1] < got[0, -2] * 6.0 and got[0, -1] == pytest.approx(acc_dt[-1], abs=1e-10 * T)  # clipped
 
 
 def test_replay_rejects_out_of_range_step_counts():
