@@ -157,6 +157,20 @@ typedef struct mkpp_amd_batch_segment {
 int mkpp_amd_device_integrate_batch(const mkpp_amd_batch_segment* segments, int num_segments, double dt, int steps,
                                     int method, double initial_step_seed, long max_attempts, int cells_per_lane,
                                     int workgroups, void* workspace, void* stream);
+/* The same persistent batch with no workspace argument and no allocation.
+ * No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The queue heads live in a static device array of
+ * MKPP_AMD_BATCH_STATIC_SCRATCH_BYTES that each GPU the library runs on holds
+ * until the process ends; the heads need MKPP_AMD_BATCH_WORKSPACE_BYTES of it
+ * whatever the input size.  Every call on one GPU shares that GPU's array and
+ * resets it on the stream before its launch, so calls that overlap on one GPU
+ * (concurrent streams or host threads) race: serialize them.  Arguments,
+ * limits and results are those of mkpp_amd_device_integrate_batch. */
+#define MKPP_AMD_BATCH_STATIC_SCRATCH_BYTES 256
+int mkpp_amd_device_integrate_batch_static(const mkpp_amd_batch_segment* segments, int num_segments, double dt,
+                                           int steps, int method, double initial_step_seed, long max_attempts,
+                                           int cells_per_lane, int workgroups, void* stream);
 /* Workgroups of the batch kernel for `method` that one CU holds at once, and
  * the CU count of the current device (the default grid is their product). */
 int mkpp_amd_batch_capacity(int method, int* per_cu, int* num_cus);

@@ -71,9 +71,10 @@ struct mkpp_handle_s {
 
   void set_error(const std::string& s) { last_error = s; }
   ~mkpp_handle_s() {
-    hipFree(d_conc); hipFree(d_temp); hipFree(d_rho); hipFree(d_par); hipFree(d_tol); hipFree(d_out);
-    hipFree(d_status); hipFree(d_acc); hipFree(d_att);
-    if (stream) hipStreamDestroy(stream);
+    for (void* p : {(void*)d_conc, (void*)d_temp, (void*)d_rho, (void*)d_par, (void*)d_tol, (void*)d_out,
+                    (void*)d_status, (void*)d_acc, (void*)d_att})
+      (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
@@ -116,7 +117,7 @@ int validate_state(mkpp_handle_s* h) {
 
 template <class T>
 int grow(mkpp_handle_s* h, T** p, size_t n) {
-  hipFree(*p);
+  (void)hipFree(*p);
   *p = nullptr;
   HIPCHK(h, hipMalloc((void**)p, n * sizeof(T) + 64));
   return MKPP_SUCCESS;
@@ -596,10 +597,13 @@ int mkpp_amd_batch_capacity(int method, int* per_cu, int* num_cus) {
   return *per_cu > 0 ? MKPP_SUCCESS : MKPP_ERROR_INVALID_STATE;
 }
 
-int mkpp_amd_device_integrate_batch(const mkpp_amd_batch_segment* segments, int num_segments, double dt, int steps,
-                                    int method, double initial_step_seed, long max_attempts, int cells_per_lane,
-                                    int workgroups, void* workspace, void* stream) {
-  if (!segments || !workspace) return MKPP_ERROR_NULL_POINTER;
+namespace {
+// Both batch entry points: `heads` is device memory for the BATCH_QUEUES
+// queue heads (the caller's workspace, or the library's static scratch).
+int batch_integrate(const mkpp_amd_batch_segment* segments, int num_segments, double dt, int steps, int method,
+                    double initial_step_seed, long max_attempts, int cells_per_lane, int workgroups,
+                    unsigned int* heads, void* stream) {
+  if (!segments || !heads) return MKPP_ERROR_NULL_POINTER;
   if (num_segments < 1 || num_segments > mkpp::BATCH_MAX_SEGMENTS || !std::isfinite(dt) || dt <= 0.0 || steps < 1 ||
       method < mkpp::ROS2 || method > mkpp::RODAS4 || !(initial_step_seed > 0.0) || cells_per_lane < 1 ||
       cells_per_lane > 64 || workgroups < 0)
@@ -656,7 +660,7 @@ int mkpp_amd_device_integrate_batch(const mkpp_amd_batch_segment* segments, int 
   }
   if (per_queue_max >= (1L << 31)) return MKPP_ERROR_INVALID_ARGUMENT;  // 32-bit queue heads
   b.nseg = num_segments;
-  b.heads = static_cast<unsigned int*>(workspace);
+  b.heads = heads;
   const mkpp::BatchLaunch* L = mkpp::kBatchLaunch[method]();
   if (workgroups == 0) {
     int dev = 0, cus = 0;
@@ -668,6 +672,66 @@ int mkpp_amd_device_integrate_batch(const mkpp_amd_batch_segment* segments, int 
     workgroups = per_cu * cus;
   }
   return L->launch(b, lds, workgroups, (hipStream_t)stream) == 0 ? MKPP_SUCCESS : MKPP_ERROR_INVALID_STATE;
+}
+}  // namespace
+
+// Scratch of mkpp_amd_device_integrate_batch_static: the queue heads, in a
+// fixed device array of MKPP_AMD_BATCH_STATIC_SCRATCH_BYTES that every GPU the
+// library runs on holds until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the
+// symbol): the device address comes from hipGetSymbolAddress, once per
+// device with that device current, and is kept per device.
+static __device__ __align__(256) unsigned int scratch[MKPP_AMD_BATCH_STATIC_SCRATCH_BYTES / sizeof(unsigned int)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+static_assert(sizeof(unsigned int) * mkpp::BATCH_QUEUES <= MKPP_AMD_BATCH_STATIC_SCRATCH_BYTES,
+              "the queue heads must fit the static scratch");
+static_assert(MKPP_AMD_BATCH_WORKSPACE_BYTES == sizeof(unsigned int) * mkpp::BATCH_QUEUES,
+              "the documented workspace size is the queue heads'");
+std::mutex g_scratch_mutex;
+unsigned int* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch (nullptr and *rc set on failure).
+unsigned int* static_scratch(int* rc) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    *rc = MKPP_ERROR_INVALID_STATE;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || ptr == nullptr) {
+      *rc = MKPP_ERROR_INVALID_STATE;
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = static_cast<unsigned int*>(ptr);
+  }
+  *rc = MKPP_SUCCESS;
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+int mkpp_amd_device_integrate_batch(const mkpp_amd_batch_segment* segments, int num_segments, double dt, int steps,
+                                    int method, double initial_step_seed, long max_attempts, int cells_per_lane,
+                                    int workgroups, void* workspace, void* stream) {
+  if (!workspace) return MKPP_ERROR_NULL_POINTER;
+  return batch_integrate(segments, num_segments, dt, steps, method, initial_step_seed, max_attempts, cells_per_lane,
+                         workgroups, static_cast<unsigned int*>(workspace), stream);
+}
+
+int mkpp_amd_device_integrate_batch_static(const mkpp_amd_batch_segment* segments, int num_segments, double dt,
+                                           int steps, int method, double initial_step_seed, long max_attempts,
+                                           int cells_per_lane, int workgroups, void* stream) {
+  // the heads this launch needs: one 32-bit counter per queue, whatever the input size
+  const size_t need = sizeof(unsigned int) * mkpp::BATCH_QUEUES;
+  if (need > MKPP_AMD_BATCH_STATIC_SCRATCH_BYTES) return MKPP_ERROR_INVALID_ARGUMENT;
+  int rc = MKPP_SUCCESS;
+  unsigned int* heads = static_scratch(&rc);
+  if (!heads) return rc;
+  return batch_integrate(segments, num_segments, dt, steps, method, initial_step_seed, max_attempts, cells_per_lane,
+                         workgroups, heads, stream);
 }
 
 // ---------------------------------------------------------------------------

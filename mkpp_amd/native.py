@@ -85,6 +85,8 @@ BATCH_WORKSPACE_BYTES = 32
 
 _SIGS["mkpp_amd_device_integrate_batch"] = ([C.POINTER(BatchSegment), C.c_int, C.c_double, C.c_int, C.c_int,
                                              C.c_double, C.c_long, C.c_int, C.c_int, _vp, _vp], C.c_int)
+_SIGS["mkpp_amd_device_integrate_batch_static"] = ([C.POINTER(BatchSegment), C.c_int, C.c_double, C.c_int, C.c_int,
+                                                    C.c_double, C.c_long, C.c_int, C.c_int, _vp], C.c_int)
 _SIGS["mkpp_amd_batch_capacity"] = ([C.c_int, _ip, _ip], C.c_int)
 _SIGS["mkpp_amd_diff_info"] = ([C.c_char_p, _ip, _ip], C.c_int)
 _SIGS["mkpp_amd_device_rate_constants"] = ([C.c_char_p, _vp, C.c_long, C.c_long, _vp, _vp, _vp, C.c_long, _vp,
