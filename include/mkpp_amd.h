@@ -80,6 +80,10 @@ int mkpp_amd_get_last_error(mkpp_handle_t handle, char* buffer, size_t buffer_le
  * keeps the current values, else num_species entries in physical order. */
 int mkpp_amd_set_solver_options(mkpp_handle_t handle, int method, const double* atol, const double* rtol,
                                 double initial_step_seed);
+/* Extension: the handle's solver options (method id, num_species atol and
+ * rtol values in physical order, seed); any pointer may be NULL. */
+int mkpp_amd_get_solver_options(mkpp_handle_t handle, int* method, double* atol, double* rtol,
+                                double* initial_step_seed);
 /* Extension: attempted-step limit per integrate() call (reference: unbounded). */
 int mkpp_amd_set_max_attempts(mkpp_handle_t handle, long max_attempts);
 /* Extension: `steps` consecutive integrate(dt) calls per cell in one launch
@@ -174,6 +178,50 @@ int mkpp_amd_device_integrate_batch_static(const mkpp_amd_batch_segment* segment
 /* Workgroups of the batch kernel for `method` that one CU holds at once, and
  * the CU count of the current device (the default grid is their product). */
 int mkpp_amd_batch_capacity(int method, int* per_cu, int* num_cus);
+
+/* ---- 6. registry execution (MechanismRegistry::execute) ----
+ * The reference's name-dispatched execute(name, state, HostExecutionParams)
+ * (src/mkpp_host/include/mkpp_host/mechanism_manager.hpp:118-209) with the
+ * per-cell semantics of its ParamsTiledCellIntegrator
+ * (src/mkpp_host/include/mkpp_host/dispatcher.hpp:80-98, 197-255): per cell,
+ * T = cell_temperature ? cell_temperature[cell] : temp, the air density
+ * likewise, and the forcing is the cell's row of cell_jvals when it is given
+ * (both extents non-zero), else the shared vector jvals, else zeros; the cell
+ * is integrated `steps` times over dt.  The handle supplies the mechanism and
+ * the solver options (mkpp_amd_set_solver_options; the per-mechanism drop-ins
+ * preset their generation-time ones).  All pointers are device pointers;
+ * conc is LayoutLeft (conc[cell + stride * species]).  The call is
+ * synchronous, as the reference's execute fences: per-cell status and step
+ * counts are then available from mkpp_amd_get_cell_status.  Returns
+ * MKPP_ERROR_INVALID_ARGUMENT for an extent that is neither 0 nor num_cells,
+ * a forcing shorter than the mechanism's runtime parameters, or a scalar
+ * temperature or density that is not positive and finite where it is used;
+ * MKPP_ERROR_CONVERGENCE if a cell reached the attempt limit.
+ * rh, clw, cell_humidity and cell_cloud_water are accepted and validated; no
+ * bundled mechanism reads them. */
+typedef struct mkpp_amd_execution_params {
+  double dt;
+  int steps;
+  const double* jvals; /* [jvals_extent] shared by every cell, or NULL */
+  long jvals_extent;
+  double temp;         /* K, where cell_temperature is NULL */
+  double air_density;  /* molec cm-3, where cell_density is NULL */
+  double rh;
+  double clw;
+  const double* cell_temperature; /* [num_cells] or NULL */
+  const double* cell_humidity;    /* [num_cells] or NULL */
+  const double* cell_density;     /* [num_cells] or NULL */
+  const double* cell_cloud_water; /* [num_cells] or NULL */
+  /* per-cell forcing: cell_jvals[cell * cell_jvals_stride0 + slot * cell_jvals_stride1] */
+  const double* cell_jvals;
+  long cell_jvals_extent0;
+  long cell_jvals_extent1;
+  long cell_jvals_stride0;
+  long cell_jvals_stride1;
+} mkpp_amd_execution_params;
+
+int mkpp_amd_execute(mkpp_handle_t handle, double* conc, long num_cells, long stride,
+                     const mkpp_amd_execution_params* params, void* stream);
 
 /* ---- 5. differentiation (wave-per-cell mechanisms, Rodas3 and Ros4) ----
  * Replaces the reference's frozen-Jacobian checkpoint / adjoint templates

@@ -86,8 +86,8 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
   // ---- environment, fixed species, rate constants (as the forward pass) ----
   if (lane < NF) ylds[NV + lane] = a.conc[cell + a.stride * (long)T_::FIXPHYS[lane]];
   if (lane == 0) ylds[W::ONE] = 1.0;
-  const double Tk = a.temp ? a.temp[cell] : 300.0;
-  const double rho = a.air_density ? a.air_density[cell] : 2.4476e19;
+  const double Tk = a.temp ? a.temp[cell] : a.temp_default;
+  const double rho = a.air_density ? a.air_density[cell] : a.rho_default;
   const double Mtb = (W::THIRD_BODY >= 0) ? a.conc[cell + a.stride * (long)(W::THIRD_BODY >= 0 ? W::THIRD_BODY : 0)] : rho;
   double kreg[NGP], kbar[NGP];
 #pragma unroll
@@ -101,7 +101,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
       } else {
         const int law = T_::LAW[gi];
         const int ph = T_::PHOTO[gi];
-        const double jv = (law == 0 && a.jvals != nullptr) ? a.jvals[cell + a.jstride * ph] : 0.0;
+        const double jv = (law == 0 && a.jvals != nullptr) ? a.jvals[cell * a.jcstride + a.jstride * ph] : 0.0;
         kreg[p] = wave_rate(law, &T_::PARAMS[gi * 7], Tk, Mtb, jv);
       }
     }
@@ -558,13 +558,13 @@ __global__ void __launch_bounds__(64) wave_rate_constants_kernel(DeviceArgs a, d
   using T_ = WaveTables<W>;
   const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= a.ncells) return;
-  const double Tk = a.temp ? a.temp[cell] : 300.0;
-  const double rho = a.air_density ? a.air_density[cell] : 2.4476e19;
+  const double Tk = a.temp ? a.temp[cell] : a.temp_default;
+  const double rho = a.air_density ? a.air_density[cell] : a.rho_default;
   const double Mtb = (W::THIRD_BODY >= 0) ? a.conc[cell + a.stride * (long)(W::THIRD_BODY >= 0 ? W::THIRD_BODY : 0)] : rho;
   for (int gi = 0; gi < W::NG; ++gi) {
     const int law = T_::LAW[gi];
     const int ph = T_::PHOTO[gi];
-    const double jv = (law == 0 && a.jvals != nullptr) ? a.jvals[cell + a.jstride * ph] : 0.0;
+    const double jv = (law == 0 && a.jvals != nullptr) ? a.jvals[cell * a.jcstride + a.jstride * ph] : 0.0;
     k[cell + kstride * (long)gi] = wave_rate(law, &T_::PARAMS[gi * 7], Tk, Mtb, jv);
   }
 }

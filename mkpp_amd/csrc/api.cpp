@@ -339,6 +339,17 @@ int mkpp_amd_set_solver_options(mkpp_handle_t handle, int method, const double* 
   return MKPP_SUCCESS;
 }
 
+int mkpp_amd_get_solver_options(mkpp_handle_t handle, int* method, double* atol, double* rtol,
+                                double* initial_step_seed) {
+  if (!handle) return MKPP_ERROR_NULL_POINTER;
+  const mkpp_handle_s* h = H(handle);
+  if (method) *method = h->method;
+  if (atol) std::copy(h->atol.begin(), h->atol.end(), atol);
+  if (rtol) std::copy(h->rtol.begin(), h->rtol.end(), rtol);
+  if (initial_step_seed) *initial_step_seed = h->seed;
+  return MKPP_SUCCESS;
+}
+
 int mkpp_amd_set_max_attempts(mkpp_handle_t handle, long max_attempts) {
   if (!handle) return MKPP_ERROR_NULL_POINTER;
   H(handle)->max_attempts = max_attempts;
@@ -407,6 +418,101 @@ int mkpp_amd_integrate_steps(mkpp_handle_t handle, double dt, int steps) {
 }
 
 int mkpp_amd_integrate(mkpp_handle_t handle, double dt) { return mkpp_amd_integrate_steps(handle, dt, 1); }
+
+// MechanismRegistry::execute semantics on device pointers
+// (mechanism_manager.hpp:118-209, dispatcher.hpp:197-255).
+int mkpp_amd_execute(mkpp_handle_t handle, double* conc, long num_cells, long stride,
+                     const mkpp_amd_execution_params* params, void* stream) {
+  if (!handle) return MKPP_ERROR_NULL_POINTER;
+  mkpp_handle_s* h = H(handle);
+  if (!conc || !params) {
+    h->set_error("FATAL ERROR: execute requires a state pointer and execution parameters");
+    return MKPP_ERROR_NULL_POINTER;
+  }
+  if (!h->initialized) {
+    h->set_error("FATAL ERROR: MKPP handle must be initialized before integration");
+    return MKPP_ERROR_INVALID_STATE;
+  }
+  const mkpp_amd_execution_params& p = *params;
+  const long P = h->ops->num_runtime_parameters;
+  auto bad = [&](const char* msg) {
+    h->set_error(std::string("FATAL ERROR: ") + msg);
+    return MKPP_ERROR_INVALID_ARGUMENT;
+  };
+  if (num_cells != (long)h->num_cells || stride < num_cells)
+    return bad("state view extents do not match the handle and mechanism");
+  if (!std::isfinite(p.dt) || p.dt <= 0.0 || p.steps < 1) return bad("dt must be positive and finite, steps >= 1");
+  const bool per_cell_forcing = p.cell_jvals && p.cell_jvals_extent0 != 0 && p.cell_jvals_extent1 != 0;
+  if (p.cell_jvals_extent0 != 0 && p.cell_jvals_extent0 != num_cells)
+    return bad("per-cell forcing extent does not match state");
+  if (per_cell_forcing && p.cell_jvals_extent1 < P) return bad("per-cell forcing has too few runtime parameters");
+  if (per_cell_forcing && (p.cell_jvals_stride0 < 0 || p.cell_jvals_stride1 < 0))
+    return bad("per-cell forcing strides must not be negative");
+  if (!per_cell_forcing && p.jvals && P > 0 && p.jvals_extent < P)
+    return bad("photolysis array has fewer entries than the mechanism's runtime parameters");
+  if (!p.cell_temperature && (!std::isfinite(p.temp) || p.temp <= 0.0))
+    return bad("temperature must be positive and finite");
+  if (!p.cell_density && (!std::isfinite(p.air_density) || p.air_density <= 0.0))
+    return bad("air density must be positive and finite");
+  if (!std::isfinite(p.rh) || !std::isfinite(p.clw)) return bad("relative humidity and cloud water must be finite");
+  int rc;
+  if ((rc = ensure_device(h)) != MKPP_SUCCESS) return rc;
+  const size_t n = h->num_cells, N = (size_t)h->ops->num_species;
+  std::vector<double> tol(2 * N);
+  for (size_t i = 0; i < N; ++i) {
+    tol[i] = h->atol[i];
+    tol[N + i] = h->rtol[i];
+  }
+  hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+  HIPCHK(h, hipMemcpyAsync(h->d_tol, tol.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemsetAsync(h->d_acc, 0, n * sizeof(int), s));
+  HIPCHK(h, hipMemsetAsync(h->d_att, 0, n * sizeof(int), s));
+  DeviceArgs a{};
+  a.conc = conc;
+  a.ncells = num_cells;
+  a.stride = stride;
+  a.temp = p.cell_temperature;
+  a.temp_default = p.temp;
+  a.air_density = p.cell_density;
+  a.rho_default = p.air_density;
+  if (P > 0 && per_cell_forcing) {
+    a.jvals = p.cell_jvals;
+    a.jcstride = p.cell_jvals_stride0;
+    a.jstride = p.cell_jvals_stride1;
+  } else if (P > 0 && p.jvals) {
+    a.jvals = p.jvals;
+    a.jcstride = 0;
+    a.jstride = 1;
+  }
+  a.dt = p.dt;
+  a.steps = p.steps;
+  a.method = h->method;
+  a.seed = h->seed;
+  a.atol = h->d_tol;
+  a.rtol = h->d_tol + N;
+  a.status = h->d_status;
+  a.accepted = h->d_acc;
+  a.attempted = h->d_att;
+  a.max_attempts = h->max_attempts;
+  if (h->ops->integrate(a, s) != 0) {
+    h->set_error("FATAL ERROR: integrate kernel launch failed");
+    return MKPP_ERROR_INVALID_STATE;
+  }
+  h->status.resize(n);
+  h->accepted.resize(n);
+  h->attempted.resize(n);
+  HIPCHK(h, hipMemcpyAsync(h->status.data(), h->d_status, n * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipMemcpyAsync(h->accepted.data(), h->d_acc, n * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipMemcpyAsync(h->attempted.data(), h->d_att, n * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  for (size_t c = 0; c < n; ++c) {
+    if (h->status[c] == mkpp::ERR_CONVERGENCE_FAILURE) {
+      h->set_error("FATAL ERROR: step-size control exceeded the attempted-step limit in cell " + std::to_string(c));
+      return MKPP_ERROR_CONVERGENCE;
+    }
+  }
+  return MKPP_SUCCESS;
+}
 
 int mkpp_amd_get_cell_status(mkpp_handle_t handle, int* status, int* accepted, int* attempted) {
   if (!handle) return MKPP_ERROR_NULL_POINTER;

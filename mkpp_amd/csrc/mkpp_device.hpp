@@ -32,15 +32,20 @@ enum CellErrorCode : int {
 enum Method : int { ROS2 = 0, ROS3 = 1, ROS4 = 2, RODAS3 = 3, RODAS4 = 4 };
 
 // Arguments of one integrate launch.  Layout is the C-ABI's LayoutLeft:
-// conc[cell + stride * species], jvals[cell + jstride * slot].
+// conc[cell + stride * species], jvals[cell * jcstride + jstride * slot]
+// (jcstride 1: the C ABI's per-cell LayoutLeft forcing; 0: one vector shared
+// by every cell; any other value: a strided [cells x slots] view).
 struct DeviceArgs {
   double* conc;
   long ncells;
   long stride;
-  const double* temp;         // [ncells] or null (300 K)
-  const double* air_density;  // [ncells] or null (2.4476e19)
-  const double* jvals;        // [NUM_RUNTIME_PARAMETERS x jstride] or null (zeros)
+  const double* temp;         // [ncells] or null (temp_default)
+  const double* air_density;  // [ncells] or null (rho_default)
+  const double* jvals;        // rate parameters (see above) or null (zeros)
   long jstride;
+  long jcstride = 1;
+  double temp_default = 300.0;
+  double rho_default = 2.4476e19;
   double dt;
   int steps;
   int method;
@@ -441,9 +446,9 @@ __device__ __forceinline__ void integrate_lane_cell(const DeviceArgs& a, long ce
   for (int q = 0; q < M::NF; ++q) yf[q] = (q < M::NFIXED) ? a.conc[cell + a.stride * M::fixed_phys(q)] : 0.0;
 #pragma unroll
   for (int p = 0; p < M::NPHOTO; ++p)
-    jv[p] = (a.jvals != nullptr && p < M::NUM_RUNTIME_PARAMETERS) ? a.jvals[cell + a.jstride * p] : 0.0;
-  const double T = a.temp ? a.temp[cell] : 300.0;
-  const double rho = a.air_density ? a.air_density[cell] : 2.4476e19;
+    jv[p] = (a.jvals != nullptr && p < M::NUM_RUNTIME_PARAMETERS) ? a.jvals[cell * a.jcstride + a.jstride * p] : 0.0;
+  const double T = a.temp ? a.temp[cell] : a.temp_default;
+  const double rho = a.air_density ? a.air_density[cell] : a.rho_default;
   double Mtb = rho;
 #pragma unroll
   for (int q = 0; q < M::NFIXED; ++q)
@@ -492,6 +497,9 @@ struct EvalArgs {
   const double* air_density;
   const double* jvals;
   long jstride;
+  long jcstride = 1;
+  double temp_default = 300.0;
+  double rho_default = 2.4476e19;
   double* out;  // rates: [ncells x N]; jacobian: [ncells x N x N], LayoutLeft
   long out_stride;
 };
@@ -503,9 +511,9 @@ __device__ __forceinline__ void eval_setup(const EvalArgs& a, long cell, double 
   for (int i = 0; i < M::N; ++i) y[i] = a.conc[cell + a.stride * i];
 #pragma unroll
   for (int p = 0; p < M::NPHOTO; ++p)
-    jv[p] = (a.jvals != nullptr && p < M::NUM_RUNTIME_PARAMETERS) ? a.jvals[cell + a.jstride * p] : 0.0;
-  const double T = a.temp ? a.temp[cell] : 300.0;
-  const double rho = a.air_density ? a.air_density[cell] : 2.4476e19;
+    jv[p] = (a.jvals != nullptr && p < M::NUM_RUNTIME_PARAMETERS) ? a.jvals[cell * a.jcstride + a.jstride * p] : 0.0;
+  const double T = a.temp ? a.temp[cell] : a.temp_default;
+  const double rho = a.air_density ? a.air_density[cell] : a.rho_default;
   const double Mtb = M::THIRD_BODY >= 0 ? y[M::THIRD_BODY >= 0 ? M::THIRD_BODY : 0] : rho;
   M::rate_constants(T, Mtb, jv, k, dk);
 }

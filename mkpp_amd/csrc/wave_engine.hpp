@@ -224,8 +224,8 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
   }
   if (lane < W::NF) ylds[NV + lane] = a.conc[cell + a.stride * (long)T_::FIXPHYS[lane]];
   if (lane == 0) ylds[W::ONE] = 1.0;
-  const double Tk = a.temp ? a.temp[cell] : 300.0;
-  const double rho = a.air_density ? a.air_density[cell] : 2.4476e19;
+  const double Tk = a.temp ? a.temp[cell] : a.temp_default;
+  const double rho = a.air_density ? a.air_density[cell] : a.rho_default;
   const double Mtb = (W::THIRD_BODY >= 0) ? a.conc[cell + a.stride * (long)(W::THIRD_BODY >= 0 ? W::THIRD_BODY : 0)] : rho;
 
   // non-finite pre-check (dispatcher.hpp:289-307) when a status array is given
@@ -249,7 +249,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
     if (g < NG) {
       const int law = T_::LAW[g];
       const int ph = T_::PHOTO[g];
-      const double jv = (law == 0 && a.jvals != nullptr) ? a.jvals[cell + a.jstride * ph] : 0.0;
+      const double jv = (law == 0 && a.jvals != nullptr) ? a.jvals[cell * a.jcstride + a.jstride * ph] : 0.0;
       kreg[p] = wave_rate(law, &T_::PARAMS[g * 7], Tk, Mtb, jv);
     }
   }
@@ -599,8 +599,8 @@ __global__ void __launch_bounds__(64) wave_eval_kernel(EvalArgs a, int what) {
   using T_ = WaveTables<W>;
   const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= a.ncells) return;
-  const double Tk = a.temp ? a.temp[cell] : 300.0;
-  const double rho = a.air_density ? a.air_density[cell] : 2.4476e19;
+  const double Tk = a.temp ? a.temp[cell] : a.temp_default;
+  const double rho = a.air_density ? a.air_density[cell] : a.rho_default;
   const double Mtb = (W::THIRD_BODY >= 0) ? a.conc[cell + a.stride * (long)(W::THIRD_BODY >= 0 ? W::THIRD_BODY : 0)] : rho;
   auto y = [&](int phys) { return T_::UNIT[phys] ? 1.0 : a.conc[cell + a.stride * phys]; };
   if (what == 0) {
@@ -610,7 +610,7 @@ __global__ void __launch_bounds__(64) wave_eval_kernel(EvalArgs a, int what) {
   }
   for (int g = 0; g < W::NG; ++g) {
     const int law = T_::LAW[g];
-    const double jv = (law == 0 && a.jvals != nullptr) ? a.jvals[cell + a.jstride * T_::PHOTO[g]] : 0.0;
+    const double jv = (law == 0 && a.jvals != nullptr) ? a.jvals[cell * a.jcstride + a.jstride * T_::PHOTO[g]] : 0.0;
     const double* P = &T_::PARAMS[g * 7];
     const double k = wave_rate(law, P, Tk, Mtb, jv);
     const int np = T_::GNPOW_FULL[g];

@@ -6,10 +6,13 @@ Writes, per bundled mechanism:
   mkpp_amd/csrc/gen/<mech>.hip    HIP/CDNA4 kernels
   mkpp_amd/artifacts/<mech>_manifest.json, <mech>/factorization_plan.{json,md}
                                   the reference's on-disk artifacts
+  include/mechanisms/<mech>.hpp, <mech>/mkpp.h
+                                  SolverKernels and C headers (reference defaults)
 and the persistent batch units over all of them (mkpp_amd/csrc/gen/batch_*).
 """
 
 import argparse
+import json
 import os
 import sys
 
@@ -57,6 +60,15 @@ def main(argv=None):
     if compiled:
         names = [m for m in BUNDLED if m in compiled]
         registry(names)
+        from . import emit_host
+
+        inc = os.path.join(root, "include")
+        for nm in names:
+            with open(os.path.join(root, "mkpp_amd", "plans", f"{nm}.json")) as f:
+                p = json.load(f)
+            # the in-tree headers carry the reference's defaults (ROS-3, 1e-3, 1e-6, 1e-6)
+            emit_host.write_mechanism(p, emit_host.SolverDefaults(), os.path.join(inc, "mechanisms"))
+        emit_host.write_mechanisms_list(names, os.path.join(inc, "mkpp_host"))
         from . import emit_batch
 
         emit_batch.write([engines[m] for m in names if m in BATCHED], os.path.join(root, "mkpp_amd", "csrc", "gen"))

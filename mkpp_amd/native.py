@@ -60,6 +60,8 @@ _SIGS = {
     "mkpp_amd_get_last_error": ([_vp, C.c_char_p, _sz], C.c_int),
     "mkpp_amd_set_solver_options": ([_vp, C.c_int, _dp, _dp, C.c_double], C.c_int),
     "mkpp_amd_set_max_attempts": ([_vp, C.c_long], C.c_int),
+    "mkpp_amd_get_solver_options": ([_vp, _ip, _dp, _dp, _dp], C.c_int),
+    "mkpp_amd_execute": ([_vp, _vp, C.c_long, C.c_long, _vp, _vp], C.c_int),
     "mkpp_amd_get_cell_status": ([_vp, _ip, _ip, _ip], C.c_int),
     "mkpp_amd_device_integrate": ([C.c_char_p, _vp, C.c_long, C.c_long, _vp, _vp, _vp, C.c_long, C.c_double, C.c_int,
                                    C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, C.c_long, _vp], C.c_int),
