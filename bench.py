@@ -40,7 +40,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cells", type=int, default=262144)
     ap.add_argument("--cpu-sample", type=int, default=8192, help="cells timed on the host (0 = skip)")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host threads of the CPU baseline (0 = the physical cores this process may use, at most 16)")
+    ap.add_argument("--cpu-leg", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--no-configs", dest="configs", action="store_false",
+                    help="skip the chapman fixed-step and small_strato Ros4 configurations")
     ap.add_argument("--no-batch", dest="batch", action="store_false",
                     help="skip the mixed-mechanism persistent-vs-plain comparison reported as batching")
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
@@ -55,6 +59,10 @@ def main():
                     help="only run a 256 MiB elementwise copy (x * 1.0) three times: the known-size kernel "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
+    if args.cpu_leg:
+        print(json.dumps(cpu_leg(json.loads(args.cpu_leg))))
+        return
+    args.cpu_threads = args.cpu_threads or cpu_threads_default()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): cells are independent, "
               "so N GPUs run N disjoint cell ranges with no communication", file=sys.stderr)
@@ -157,20 +165,14 @@ def main():
 
     cpu = None
     if args.cpu_sample > 0:
-        from oracle.oracle import Oracle
-
-        o = Oracle("saprc99")
-        m = min(args.cpu_sample, ncell)
-        c = np.ascontiguousarray(w["conc"][:, :m]).copy()
-        t1 = time.perf_counter()
-        o.integrate(c, w["dt"], 1, "rodas3", temp=w["temp"][:m], air_density=w["air_density"][:m],
-                    jvals=np.ascontiguousarray(w["rate_parameters"][:, :m]), atol=w["atol"], rtol=w["rtol"],
-                    seed=w["seed"], hoist=False, threads=args.cpu_threads)
-        cpu_s = time.perf_counter() - t1
-        progress(f"cpu baseline: {m} cells in {cpu_s:.1f} s")
-        cpu = {"value": m / cpu_s, "unit": "grid-cells/s", "cores": args.cpu_threads, "kind": "port",
-               "sample": f"first {m} of the {ncell} cells, same inputs, 1 h Rodas3 rtol 1e-3, "
-                         f"{cpu_s:.1f} s wall, OpenMP static(64) over cells, rate constants re-evaluated per term"}
+        cpu = cpu_baseline({"workload": "saprc99_urban", "cells": ncell, "sample": min(args.cpu_sample, ncell)},
+                           args.cpu_threads, single_thread_sample=512, hoisted=True)
+        progress(f"cpu baseline: {cpu['value']:.0f} cells/s on {cpu['cores']} cores")
+    configs = None
+    if args.configs:
+        configs = {"chapman_fixed_step": run_config(np, "chapman_fixed", 4096, args.cpu_threads),
+                   "small_strato_ros4": run_config(np, "small_strato", 65536, args.cpu_threads, cpu_sample=16384)}
+        progress("chapman and small_strato configurations done")
 
     value = ncell / (ms_per_step / 1e3)
     out = {
@@ -192,8 +194,8 @@ def main():
         "kernel_ms": float(np.mean(kernel_ms)),
         "accepted_steps_mean": float(accepted.mean()),
         "attempted_steps_mean": float(attempted.mean()),
-        "roofline": {"bound": "mfma", "pipe": "fp64 (VALU)", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": traffic,
+        "roofline": {"bound": "valu", "pipe": "fp64 vector ALU (no MFMA)", "achieved": achieved / 1e12,
+                     "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": traffic,
                      "flops_per_attempted_step": step_flops},
         "cpu_baseline": cpu,
         "pcie_inclusive": pcie,
@@ -201,10 +203,167 @@ def main():
         "adjoint": adjoint,
         "saprcnov": saprcnov,
         "reduced": reduced,
+        "configs": configs,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
     print(json.dumps(out))
+
+
+def host_cpu():
+    """(model name, physical cores this process may run on, logical CPUs)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    model, cores, cur = "unknown", set(), {}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f.read().split("\n") + [""]:
+                if not line.strip():
+                    if cur.get("processor") is not None and int(cur["processor"]) in cpus:
+                        cores.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                cur[k.strip()] = v.strip()
+                if k.strip() == "model name":
+                    model = v.strip()
+    except OSError:
+        pass
+    return model, len(cores) or len(cpus), len(cpus)
+
+
+# A 1-GPU box grants its process a share of 16 host CPUs.
+HOST_CPU_SHARE = 16
+
+
+def cpu_threads_default():
+    _, cores, _ = host_cpu()
+    return max(1, min(cores, HOST_CPU_SHARE))
+
+
+def cpu_leg(spec: dict) -> dict:
+    """Runs in a child process with OMP_NUM_THREADS / OMP_PROC_BIND / OMP_PLACES
+    set by cpu_baseline(): one warm-up and `reps` timed integrations of the
+    same sample from a fresh copy of its initial state (the copy is outside the
+    timer, as the reference runner's steady-state boundary); returns the
+    median."""
+    import numpy as np
+
+    from mkpp_amd import workloads
+    from oracle.oracle import Oracle
+
+    w = getattr(workloads, spec["workload"])(spec["cells"])
+    m = spec["sample"]
+    o = Oracle(w["mechanism"])
+    jv = np.ascontiguousarray(w["rate_parameters"][:, :m]) if o.nphoto else None
+    times = []
+    for r in range(1 + spec.get("reps", 3)):
+        c = np.ascontiguousarray(w["conc"][:, :m]).copy()
+        t = time.perf_counter()
+        o.integrate(c, w["dt"], w["steps"], w["method"], temp=w["temp"][:m], air_density=w["air_density"][:m],
+                    jvals=jv, atol=w["atol"], rtol=w["rtol"], seed=w["seed"], hoist=spec.get("hoist", False),
+                    max_attempts=spec.get("max_attempts", 0), threads=spec["threads"])
+        if r > 0:
+            times.append(time.perf_counter() - t)
+    med = float(np.median(times))
+    return {"seconds": times, "median_s": med, "cells_per_s": m / med}
+
+
+def _cpu_child(spec: dict, threads: int) -> dict:
+    import subprocess
+
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    spec = dict(spec, threads=threads)
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg", json.dumps(spec)], env=env,
+                       capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline child failed: {r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().split("\n")[-1])
+
+
+def cpu_baseline(spec: dict, threads: int = 0, single_thread_sample: int = 0, hoisted: bool = False) -> dict:
+    """The C oracle (the reference algorithm restated in the reference's cost
+    structure: rate constants re-evaluated inside every F and J term) over a
+    sample of the workload, OpenMP schedule(static, 64) over cells on the
+    physical cores of this host (OMP_PROC_BIND=close, OMP_PLACES=cores), the
+    median of 3 timed runs after a warm-up.  Optionally the same on one
+    thread, and with the rate constants hoisted once per cell and call."""
+    model, phys, logical = host_cpu()
+    threads = threads or cpu_threads_default()
+    main = _cpu_child(spec, threads)
+    out = {"value": main["cells_per_s"], "unit": "grid-cells/s", "cores": threads, "kind": "port",
+           "cpu_model": model, "physical_cores_visible": phys, "logical_cpus_visible": logical,
+           "omp": "OMP_PROC_BIND=close OMP_PLACES=cores, schedule(static, 64) over cells",
+           "timing": "1 warm-up + 3 timed runs, median; state copied outside the timer",
+           "sample": f"first {spec['sample']} of the {spec['cells']} cells of {spec['workload']}, "
+                     f"median {main['median_s']:.2f} s per run",
+           "runs_s": main["seconds"]}
+    if single_thread_sample:
+        one = _cpu_child(dict(spec, sample=min(single_thread_sample, spec["sample"])), 1)
+        out["single_thread"] = {"value": one["cells_per_s"], "sample_cells": min(single_thread_sample, spec["sample"]),
+                                "runs_s": one["seconds"]}
+    if hoisted:
+        h = _cpu_child(dict(spec, hoist=True), threads)
+        out["hoisted_rate_constants"] = {"value": h["cells_per_s"], "cores": threads, "runs_s": h["seconds"],
+                                         "what": "rate constants evaluated once per cell and call (not the "
+                                                 "reference's cost structure; reported separately)"}
+    return out
+
+
+def run_config(np, workload: str, cells: int, cpu_threads: int, reps: int = 5, cpu_sample: int = 0):
+    """One of the plain-method configurations on its own cells: device-resident
+    throughput over `reps` timed launches (HIP events), the FP64 rate of the
+    kernel against the peak, and the CPU port on the same cells."""
+    import torch
+
+    from mkpp_amd import native, workloads
+
+    w = getattr(workloads, workload)(cells)
+    mech = w["mechanism"]
+    plan = json.load(open(os.path.join(ROOT, "mkpp_amd", "plans", f"{mech}.json")))
+    step_flops = plan["step_flops"][w["method"]]
+    dev = torch.device("cuda:0")
+    n = w["conc"].shape[0]
+    init = torch.tensor(w["conc"], device=dev).contiguous()
+    rp = torch.tensor(w["rate_parameters"], device=dev).contiguous()
+    T = torch.tensor(w["temp"], device=dev)
+    rho = torch.tensor(w["air_density"], device=dev)
+    at = torch.full((n,), float(w["atol"]), dtype=torch.float64, device=dev)
+    rt = torch.full((n,), float(w["rtol"]), dtype=torch.float64, device=dev)
+    st = torch.zeros(cells, dtype=torch.int32, device=dev)
+    acc = torch.zeros(cells, dtype=torch.int32, device=dev)
+    att = torch.zeros(cells, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def launch(buf):
+        native.device_integrate(mech, buf, w["dt"], w["steps"], w["method"], T, rho, rp, at, rt, w["seed"],
+                                status=st, accepted=acc, attempted=att, stream=stream)
+
+    launch(init.clone())
+    bufs = [init.clone() for _ in range(reps)]
+    acc.zero_()
+    att.zero_()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for i in range(reps):
+        ev[i][0].record(stream)
+        launch(bufs[i])
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    ms = float(np.median([a.elapsed_time(b) for a, b in ev]))
+    attempted = att.cpu().numpy().astype(np.float64) / reps
+    achieved = float(attempted.sum()) * step_flops / (ms / 1e3)
+    stv = st.cpu().numpy()
+    cpu = cpu_baseline({"workload": workload, "cells": cells, "sample": cpu_sample or cells}, cpu_threads)
+    desc = {"chapman_fixed": "chapman 60 x fixed 60 s Rodas3 steps (seed 1, non-rejecting tolerances)",
+            "small_strato": "small_strato Ros4 1 h adaptive, rtol 1e-3 atol 1e3"}[workload]
+    return {"workload": f"{desc}, {cells} cells", "value": cells / (ms / 1e3), "unit": "grid-cells/s",
+            "kernel_ms": ms, "attempted_steps_mean": float(attempted.mean()),
+            "status_counts": {str(int(k)): int(v) for k, v in zip(*np.unique(stv, return_counts=True))},
+            "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                         "frac": achieved / PEAK_FP64, "flops_per_attempted_step": step_flops,
+                         "note": f"{cells} cells fill {cells / 128:.0f} workgroups of 128 lanes on 256 CUs: the "
+                                 "launch is occupancy- and latency-bound, not VALU-bound"},
+            "cpu_baseline": cpu, "gpu_over_cpu": (cells / (ms / 1e3)) / cpu["value"]}
 
 
 def run_saprcnov(np, cells, cpu_sample, cpu_threads, reps=2, max_attempts=5000):
@@ -255,18 +414,8 @@ def run_saprcnov(np, cells, cpu_sample, cpu_threads, reps=2, max_attempts=5000):
     achieved = float(attempted.sum()) * step_flops / (ms / 1e3)
     cpu = None
     if cpu_sample > 0:
-        from oracle.oracle import Oracle
-
-        m = min(cpu_sample, cells)
-        c = np.ascontiguousarray(w["conc"][:, :m]).copy()
-        t1 = time.perf_counter()
-        Oracle("saprcnov").integrate(c, w["dt"], 1, "rodas3", temp=w["temp"][:m], air_density=w["air_density"][:m],
-                                     jvals=np.ascontiguousarray(w["rate_parameters"][:, :m]), atol=w["atol"],
-                                     rtol=w["rtol"], seed=w["seed"], hoist=False, max_attempts=max_attempts,
-                                     threads=cpu_threads)
-        cpu_s = time.perf_counter() - t1
-        cpu = {"value": m / cpu_s, "unit": "grid-cells/s", "cores": cpu_threads, "kind": "port",
-               "sample": f"first {m} cells, {cpu_s:.1f} s wall"}
+        cpu = cpu_baseline({"workload": "saprcnov_urban", "cells": cells, "sample": min(cpu_sample, cells),
+                            "max_attempts": max_attempts}, cpu_threads)
     return {"workload": f"saprcnov Rodas3 1 h rtol 1e-3 atol 1e3, {cells} cells (saprc99 urban setup, "
                         "matched photolysis, half at night)",
             "value": cells / (ms / 1e3), "unit": "grid-cells/s", "kernel_ms": ms,

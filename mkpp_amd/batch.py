@@ -67,8 +67,8 @@ class BatchRun:
                                     g["rate_parameters"], g["atol"], g["rtol"], self.seed, status=g["status"],
                                     accepted=g["accepted"], attempted=g["attempted"], stream=stream)
 
-    def timed(self, fn, repeats: int = 2) -> float:
-        """Mean milliseconds of fn() over `repeats` fresh-state runs (HIP events)."""
+    def timed(self, fn, repeats: int = 3) -> float:
+        """Median milliseconds of fn() over `repeats` fresh-state runs (HIP events)."""
         torch = self.torch
         s = torch.cuda.current_stream()
         ms = []
@@ -81,7 +81,7 @@ class BatchRun:
             b.record(s)
             torch.cuda.synchronize()
             ms.append(a.elapsed_time(b))
-        return float(np.mean(ms))
+        return float(np.median(ms))
 
     def accepted(self) -> dict:
         return {g["mechanism"]: g["accepted"].cpu().numpy() for g in self.segments}
@@ -90,15 +90,21 @@ class BatchRun:
         return np.concatenate([g["status"].cpu().numpy() for g in self.segments])
 
 
-def autotune(run: BatchRun, cells_per_lane=(1, 2, 4), grid_fractions=(1.0, 0.75)) -> dict:
-    """Time each (cells per lane, workgroups) candidate once on `run` and return
-    the fastest with the whole table."""
+def autotune(run: BatchRun, cells_per_lane=(1, 2, 4), grid_fractions=(1.0, 0.875, 0.75, 0.5),
+             repeats: int = 3) -> dict:
+    """Time each (cells per lane, workgroups) candidate on `run` (median of
+    `repeats` fresh-state runs) and return the fastest with the whole table.
+    A candidate within 0.5 % of the fastest counts as tied; ties go to the
+    fewest cells per lane and the largest grid (the defaults), so timing noise
+    does not pick a setting."""
     per_cu, cus = native.batch_capacity(run.method)
     table = []
     for m in cells_per_lane:
         for f in grid_fractions:
             wg = max(8, int(round(per_cu * cus * f / 8.0)) * 8)
-            ms = run.timed(lambda: run.persistent(m, wg), repeats=1)
+            ms = run.timed(lambda: run.persistent(m, wg), repeats=repeats)
             table.append({"cells_per_lane": m, "workgroups": wg, "ms": ms})
-    best = min(table, key=lambda r: r["ms"])
-    return {"best": best, "table": table, "resident_per_cu": per_cu, "cus": cus}
+    fastest = min(r["ms"] for r in table)
+    tied = [r for r in table if r["ms"] <= fastest * 1.005]
+    best = min(tied, key=lambda r: (r["cells_per_lane"], -r["workgroups"]))
+    return {"best": best, "table": table, "resident_per_cu": per_cu, "cus": cus, "repeats": repeats}
