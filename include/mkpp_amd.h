@@ -263,6 +263,23 @@ int mkpp_amd_device_adjoint(const char* mechanism, const double* conc, long num_
                             const double* hs, const int* nsteps, int max_steps, double* lambda, double* kbar,
                             void* stream);
 
+/* Tangent-linear model along the recorded steps (forward-mode derivative of
+ * the same frozen-step map the adjoint differentiates; the reference's
+ * frozen-Jacobian counterpart is integrate_tlm.j2:33-215).  conc: in y(0),
+ * out y(T) (the replay of the recorded steps, as
+ * mkpp_amd_device_integrate_checkpointed with replay = 1 computes it);
+ * dy[cell + stride*species]: in the direction dy(0) (fixed species
+ * included), out dy(T); dk[cell + k_stride*group] the direction of the rate
+ * constants, or NULL for 0.  k as for the adjoint (NULL: rate laws).
+ * status (may be NULL): MKPP_CELL_ERR_CHECKPOINT_OVERFLOW for a step count
+ * outside [0, max_steps].  Device pointers; enqueued on `stream`; allocates
+ * nothing. */
+int mkpp_amd_device_tangent_linear(const char* mechanism, double* conc, long num_cells, long stride,
+                                   const double* temp, const double* air_density, const double* rate_parameters,
+                                   long parameter_stride, const double* k, long k_stride, int method,
+                                   const double* hs, const int* nsteps, int max_steps, double* dy, const double* dk,
+                                   int* status, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -551,6 +551,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   if (cell < a.ncells) wave_integrate_cell<W, Tab, MODE>(a, cell, lds, threadIdx.x, &g);
 }
 
+// Tangent-linear replay (MODE 4): more registers than the forward kernel (the
+// tangent stage vectors), so no occupancy target is forced on it.
+template <class W, class Tab>
+__global__ void __launch_bounds__(64) wave_tlm_kernel(DeviceArgs a, AdjArgs g, long base) {
+  extern __shared__ double lds[];
+  const long cell = wave_cell_of_block(base);
+  if (cell < a.ncells) wave_integrate_cell<W, Tab, 4>(a, cell, lds, threadIdx.x, &g);
+}
+
 // Rate constants per flux group and cell (the values the laws give; the
 // differentiable inputs of the adjoint).
 template <class W>
@@ -569,8 +578,9 @@ __global__ void __launch_bounds__(64) wave_rate_constants_kernel(DeviceArgs a, d
   }
 }
 
-// mode 0: rate constants only; 1: checkpointed forward; 2: replay; 3: adjoint.
-// Only Rodas3 and Ros4 are instantiated for differentiation.
+// mode 0: rate constants only; 1: checkpointed forward; 2: replay; 3: adjoint;
+// 4: tangent-linear replay.  Only Rodas3 and Ros4 are instantiated for
+// differentiation.
 template <class W>
 int launch_wave_diff(int mode, const DeviceArgs& a, const AdjArgs& g, hipStream_t s) {
   if (a.ncells <= 0) return 0;
@@ -582,6 +592,7 @@ int launch_wave_diff(int mode, const DeviceArgs& a, const AdjArgs& g, hipStream_
 #define MKPP_DIFF_CASE(TAB)                                                                              \
   if (mode == 1) wave_record_kernel<W, TAB, 1><<<grid, 64, fwd_lds, s>>>(a, g, base);                    \
   else if (mode == 2) wave_record_kernel<W, TAB, 2><<<grid, 64, fwd_lds, s>>>(a, g, base);               \
+  else if (mode == 4) wave_tlm_kernel<W, TAB><<<grid, 64, sizeof(double) * wave_tlm_lds_doubles<W>(), s>>>(a, g, base); \
   else wave_adjoint_kernel<W, TAB><<<grid, 64, sizeof(double) * adjoint_lds_doubles<W, TAB>(), s>>>(a, g, base);
   if (a.method != RODAS3 && a.method != ROS4) return -1;
   for (long base = 0; base < a.ncells; base += WAVE_CHUNK_CELLS) {

@@ -955,3 +955,29 @@ int mkpp_amd_device_adjoint(const char* mechanism, const double* conc, long num_
   g.kbar = kbar;
   return o->diff(3, a, g, (hipStream_t)stream) == 0 ? MKPP_SUCCESS : MKPP_ERROR_INVALID_STATE;
 }
+
+int mkpp_amd_device_tangent_linear(const char* mechanism, double* conc, long num_cells, long stride,
+                                   const double* temp, const double* air_density, const double* rate_parameters,
+                                   long parameter_stride, const double* k, long k_stride, int method,
+                                   const double* hs, const int* nsteps, int max_steps, double* dy, const double* dk,
+                                   int* status, void* stream) {
+  int rc;
+  const MechOps* o = diff_mechanism(mechanism, &rc);
+  if (!o) return rc;
+  if (!conc || !hs || !nsteps || !dy) return MKPP_ERROR_NULL_POINTER;
+  if (num_cells < 0 || stride < num_cells || (rate_parameters && parameter_stride < num_cells) ||
+      ((k || dk) && k_stride < num_cells) || max_steps < 1 || (method != mkpp::RODAS3 && method != mkpp::ROS4))
+    return MKPP_ERROR_INVALID_ARGUMENT;
+  DeviceArgs a = diff_args(conc, num_cells, stride, temp, air_density, rate_parameters, parameter_stride);
+  a.method = method;
+  a.status = status;
+  mkpp::AdjArgs g{};
+  g.k = k;
+  g.kstride = k_stride;
+  g.hs = const_cast<double*>(hs);
+  g.nsteps = const_cast<int*>(nsteps);
+  g.maxsteps = max_steps;
+  g.dy = dy;
+  g.dk = dk;
+  return o->diff(4, a, g, (hipStream_t)stream) == 0 ? MKPP_SUCCESS : MKPP_ERROR_INVALID_STATE;
+}

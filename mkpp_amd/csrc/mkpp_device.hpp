@@ -73,6 +73,8 @@ struct AdjArgs {
   int maxsteps;
   double* lambda;  // adjoint: [N x stride] physical order; in dL/dy(T), out dL/dy(0)
   double* kbar;    // adjoint: [NG x kstride] out dL/dk
+  double* dy;         // tangent-linear: [N x stride] physical order; in dy(0), out dy(T)
+  const double* dk;   // tangent-linear: [NG x kstride] direction of the rate constants, or null (0)
 };
 
 // ---------------------------------------------------------------------------

@@ -131,23 +131,16 @@ __device__ __forceinline__ void wave_zero_lu(double* lu, int lane) {
 // the flux groups) grouped by chunks, a difference of rounding only.  No
 // loop bound depends on data, so every table load and LDS gather of a step
 // can be issued at once.
-template <class W, int NB, int NGP>
-__device__ __forceinline__ void wave_eval_f(const double* __restrict__ ylds, double* __restrict__ scr,
-                                            const double (&kreg)[NGP], const bool (&valid)[NB], double (&f)[NB],
-                                            int lane) {
+//
+// wave_row_sums is steps 2 and 3 alone: any per-group values in scr[0, NG)
+// (with scr[NG] = 0.0) become sum_g nu_ig * scr[g] per row, the F sums'
+// order and grouping.  The tangent-linear mode uses it for J v and (dJ) K.
+template <class W, int NB>
+__device__ __forceinline__ void wave_row_sums(double* __restrict__ scr, const bool (&valid)[NB], double (&f)[NB],
+                                              int lane) {
   using T_ = WaveTables<W>;
   using C_ = typename T_::Code;
-  constexpr int NG = W::NG;
   static_assert(C_::PBASE + C_::NFP * 64 <= W::NSCR + W::NV + 1, "F partial slots exceed the scratch and row-sum areas");
-#pragma unroll
-  for (int p = 0; p < NGP; ++p) {
-    const int g = p * 64 + lane;
-    if (g < NG) {
-      const uint8_t* fa = &T_::GFAC[g * 3];
-      scr[g] = kreg[p] * (((1.0 * ylds[fa[0]]) * ylds[fa[1]]) * ylds[fa[2]]);
-    }
-  }
-  if (lane == 0) scr[NG] = 0.0;
 #pragma unroll
   for (int p = 0; p < C_::NFP; ++p) {
     const uint32_t* tw = &T_::CF_TERM[(p * 64 + lane) * (C_::TF / 2)];
@@ -168,6 +161,137 @@ __device__ __forceinline__ void wave_eval_f(const double* __restrict__ ylds, dou
     for (int c = 1; c < C_::FMAXC[b]; ++c) sum = sum + scr[pp[c]];
     f[b] = valid[b] ? sum : 0.0;
   }
+}
+
+template <class W, int NB, int NGP>
+__device__ __forceinline__ void wave_eval_f(const double* __restrict__ ylds, double* __restrict__ scr,
+                                            const double (&kreg)[NGP], const bool (&valid)[NB], double (&f)[NB],
+                                            int lane) {
+  using T_ = WaveTables<W>;
+  constexpr int NG = W::NG;
+#pragma unroll
+  for (int p = 0; p < NGP; ++p) {
+    const int g = p * 64 + lane;
+    if (g < NG) {
+      const uint8_t* fa = &T_::GFAC[g * 3];
+      scr[g] = kreg[p] * (((1.0 * ylds[fa[0]]) * ylds[fa[1]]) * ylds[fa[2]]);
+    }
+  }
+  if (lane == 0) scr[NG] = 0.0;
+  wave_row_sums<W>(scr, valid, f, lane);
+}
+
+// Tangent-linear group values (MODE 4 of wave_integrate_cell).  Fluxes are
+// mass-action monomials r_g = k_g prod_e y_e^q_e over at most three entries.
+// Derivatives of the monomial at the state y (by y slot): d1[e] = d/dy_e,
+// d2[e][f] = d2/dy_e dy_f (the power rule on each entry).
+template <class W>
+__device__ __forceinline__ void wave_monomial_derivs(const double* __restrict__ y, int g, double& mono,
+                                                     double (&d1)[3], double (&d2)[3][3]) {
+  using T_ = WaveTables<W>;
+  const uint8_t* sl = &T_::GPOWS[g * 3];
+  const uint8_t* ql = &T_::GPOWP[g * 3];
+  double pw[3], pm1[3], pm2[3];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const double v = y[sl[e]];
+    const double v2 = v * v;
+    const int q = ql[e];
+    pw[e] = q == 0 ? 1.0 : (q == 1 ? v : (q == 2 ? v2 : v2 * v));
+    pm1[e] = q == 0 ? 0.0 : (q == 1 ? 1.0 : (q == 2 ? 2.0 * v : 3.0 * v2));
+    pm2[e] = q <= 1 ? 0.0 : (q == 2 ? 2.0 : 6.0 * v);
+  }
+  mono = (pw[0] * pw[1]) * pw[2];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    double d = pm1[e];
+#pragma unroll
+    for (int o = 0; o < 3; ++o)
+      if (o != e) d *= pw[o];
+    d1[e] = d;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      double h;
+      if (f == e) {
+        h = pm2[e];
+#pragma unroll
+        for (int o = 0; o < 3; ++o)
+          if (o != e) h *= pw[o];
+      } else {
+        h = pm1[e] * pm1[f];
+#pragma unroll
+        for (int o = 0; o < 3; ++o)
+          if (o != e && o != f) h *= pw[o];
+      }
+      d2[e][f] = h;
+    }
+  }
+}
+
+// scr[g] = dr_g = dk_g mono_g(Y) + k_g sum_e d1_e(Y) dY_e: with wave_row_sums,
+// J(Y) dY + F_k(Y) dk.  y, dy by y slot (dy of ONE is 0).
+template <class W, int NGP>
+__device__ __forceinline__ void wave_tlm_group_jvp(const double* __restrict__ y, const double* __restrict__ dy,
+                                                   double* __restrict__ scr, const double (&kreg)[NGP],
+                                                   const double (&dkreg)[NGP], int lane) {
+  using T_ = WaveTables<W>;
+  constexpr int NG = W::NG;
+#pragma unroll
+  for (int p = 0; p < NGP; ++p) {
+    const int g = p * 64 + lane;
+    if (g < NG) {
+      double mono, d1[3], d2[3][3];
+      wave_monomial_derivs<W>(y, g, mono, d1, d2);
+      const uint8_t* sl = &T_::GPOWS[g * 3];
+      double jv = 0.0;
+#pragma unroll
+      for (int e = 0; e < 3; ++e) jv = fma(d1[e], dy[sl[e]], jv);
+      scr[g] = fma(dkreg[p], mono, kreg[p] * jv);
+    }
+  }
+  if (lane == 0) scr[NG] = 0.0;
+}
+
+// scr[g] = k_g sum_e sum_f d2_ef(y) K_e dy_f + dk_g sum_e d1_e(y) K_e: with
+// wave_row_sums, the derivative of J(y) K along (dy, dk).  y, dy, K by y slot
+// (K of fixed species and ONE is 0).
+template <class W, int NGP>
+__device__ __forceinline__ void wave_tlm_group_djk(const double* __restrict__ y, const double* __restrict__ dy,
+                                                   const double* __restrict__ kv, double* __restrict__ scr,
+                                                   const double (&kreg)[NGP], const double (&dkreg)[NGP], int lane) {
+  using T_ = WaveTables<W>;
+  constexpr int NG = W::NG;
+#pragma unroll
+  for (int p = 0; p < NGP; ++p) {
+    const int g = p * 64 + lane;
+    if (g < NG) {
+      double mono, d1[3], d2[3][3];
+      wave_monomial_derivs<W>(y, g, mono, d1, d2);
+      const uint8_t* sl = &T_::GPOWS[g * 3];
+      double kk[3], dd[3];
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        kk[e] = kv[sl[e]];
+        dd[e] = dy[sl[e]];
+      }
+      double h = 0.0, jk = 0.0;
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        jk = fma(d1[e], kk[e], jk);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) h = fma(d2[e][f] * kk[e], dd[f], h);
+      }
+      scr[g] = fma(dkreg[p], jk, kreg[p] * h);
+    }
+  }
+  if (lane == 0) scr[NG] = 0.0;
+}
+
+// LDS doubles of the tangent-linear mode: the forward layout plus y_n, dy_n,
+// dY_s and K_s by y slot.
+template <class W>
+constexpr int wave_tlm_lds_doubles() {
+  return W::LDS_DOUBLES + 4 * W::NY;
 }
 
 // ---------------------------------------------------------------------------
@@ -191,8 +315,14 @@ __device__ __forceinline__ void wave_eval_f(const double* __restrict__ ylds, dou
 // locations turns frozen rows and columns into the identity's; stage
 // right-hand sides of frozen rows are zero, which with that W makes the plain
 // solves return the template's guarded zeros; dt starts at min(dt, 1 s) and is
-// multiplied by the factor after every attempt.  The mode branches are
-// compile-time; MODE 0 carries none of them.
+// multiplied by the factor after every attempt.  MODE 4: the tangent-linear
+// model along recorded steps: the replay of MODE 2 carries a tangent dx
+// (adj->dy in, out) for a direction (dy0, adj->dk) of the initial state and
+// the rate constants; per stage, W dK_s = dF_s + sum_j C_sj/h dK_j + (dJ) K_s
+// with dF_s = J(Y_s) dY_s + F_k(Y_s) dk at the stages that evaluate F, and
+// (dJ) K_s the derivative of J(y_n) K_s along (dx, dk); then
+// dx += sum_s M_s dK_s.  The mode branches are compile-time; MODE 0 carries
+// none of them.
 template <class W, class Tab, int MODE = 0>
 __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long cell, double* lds, int lane,
                                                     const AdjArgs* adj = nullptr) {
@@ -203,6 +333,12 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
   double* ylds = lu + (W::SPAN + 1);
   double* scr = ylds + W::NY;
   double* facc = scr + W::NSCR;
+  // MODE 4 only (wave_tlm_lds_doubles): y_n, dy_n, dY_s and K_s by y slot
+  double* xl = facc + (NV + 1);
+  double* dxl = xl + W::NY;
+  double* dyl = dxl + W::NY;
+  double* kvl = dyl + W::NY;
+  (void)xl; (void)dxl; (void)dyl; (void)kvl;
 
   // lane slot (b, lane) holds row ROWOF[b * 64 + lane] (255: none)
   int rb[NB], rowv[NB];
@@ -253,11 +389,39 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
       kreg[p] = wave_rate(law, &T_::PARAMS[g * 7], Tk, Mtb, jv);
     }
   }
-  if constexpr (MODE == 1 || MODE == 2) {
+  if constexpr (MODE == 1 || MODE == 2 || MODE == 4) {
     if (adj->k != nullptr) {
 #pragma unroll
       for (int p = 0; p < NGP; ++p)
         if (p * 64 + lane < NG) kreg[p] = adj->k[cell + adj->kstride * (long)(p * 64 + lane)];
+    }
+  }
+  // MODE 4: the direction of the rate constants, the tangent of the state, and
+  // the constant parts of the by-slot vectors (fixed species, ONE)
+  double dkreg[NGP], dx[NB];
+#pragma unroll
+  for (int p = 0; p < NGP; ++p) dkreg[p] = 0.0;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) dx[b] = 0.0;
+  if constexpr (MODE == 4) {
+    if (adj->dk != nullptr) {
+#pragma unroll
+      for (int p = 0; p < NGP; ++p)
+        if (p * 64 + lane < NG) dkreg[p] = adj->dk[cell + adj->kstride * (long)(p * 64 + lane)];
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int ph = valid[b] ? (int)T_::VARPHYS[rowv[b]] : 0;
+      dx[b] = valid[b] ? adj->dy[cell + a.stride * ph] : 0.0;
+    }
+    for (int i = NV + lane; i < W::NY; i += 64) {
+      const int q = i - NV;  // fixed species q, or ONE (q == NF)
+      const double v = q < W::NF ? a.conc[cell + a.stride * (long)T_::FIXPHYS[q < W::NF ? q : 0]] : 1.0;
+      const double d = q < W::NF ? adj->dy[cell + a.stride * (long)T_::FIXPHYS[q < W::NF ? q : 0]] : 0.0;
+      xl[i] = v;
+      dxl[i] = d;
+      dyl[i] = d;
+      kvl[i] = 0.0;
     }
   }
 
@@ -329,7 +493,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
     bool reject_last = false, reject_more = false;
     long attempts = 0;
     int nrep = 0;
-    if constexpr (MODE == 2) {
+    if constexpr (MODE == 2 || MODE == 4) {
       // a recorded count outside [0, maxsteps] cannot index the cell's slice
       nrep = adj->nsteps[cell];
       if (nrep < 0 || nrep > adj->maxsteps) {
@@ -337,14 +501,14 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
         nrep = 0;
       }
     }
-    while (MODE == 2 ? acc_steps < nrep : t < dt_total) {
+    while ((MODE == 2 || MODE == 4) ? acc_steps < nrep : t < dt_total) {
       if (a.max_attempts > 0 && attempts >= a.max_attempts) {
         status = ERR_CONVERGENCE_FAILURE;
         break;
       }
       ++attempts;
       ++att_steps;
-      if constexpr (MODE == 2)
+      if constexpr (MODE == 2 || MODE == 4)
         dt = adj->hs[cell * (long)adj->maxsteps + acc_steps];
       else
         dt = fmin(dt, dt_total - t);
@@ -356,6 +520,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
       int ln = lane;
       asm volatile("" : "+v"(ln));
       double Sx[NB], K[S][NB], F[NB], rdiag[NB];
+      double dK[S][NB], dF[NB];  // MODE 4 only
       bool act[NB];
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
@@ -363,6 +528,14 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
         act[b] = valid[b];
       }
       store_y(Sx);
+      if constexpr (MODE == 4) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          if (valid[b]) {
+            xl[rowv[b]] = x[b];
+            dxl[rowv[b]] = dx[b];
+          }
+      }
       // W, its LU and 1/diag.  MODE 3 runs this inside the stage loop once F1
       // (the stage-1 F, evaluated at the loop's one F site) has set the active
       // rows: a second inlined F evaluation ahead of it costs registers.
@@ -417,6 +590,22 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
             store_y(Y);
           }
           eval_rates(F);
+          if constexpr (MODE == 4) {
+            // dY_s = dx + sum_j A_sj dK_j by row, then dF_s = J(Y_s) dY_s + F_k dk
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+              double v = dx[b];
+#pragma unroll
+              for (int j = 0; j < S - 1; ++j) {
+                const double c = j < s ? Tab::A[base + j] : 0.0;
+                if (c == 0.0) continue;
+                v = (c == 1.0) ? v + dK[j][b] : v + c * dK[j][b];
+              }
+              if (valid[b]) dyl[rowv[b]] = v;
+            }
+            wave_tlm_group_jvp<W>(ylds, dyl, scr, kreg, dkreg, lane);
+            wave_row_sums<W>(scr, valid, dF, lane);
+          }
         }
         if constexpr (MODE == 3) {
           if (s == 0) {
@@ -451,6 +640,36 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
 #pragma unroll
             for (int b = 0; b < NB; ++b) K[j][b] = xs[b];
           }
+        if constexpr (MODE == 4) {
+          // (dJ) K_s at y_n, then W dK_s = dF_s + sum_j C_sj/h dK_j + (dJ) K_s
+#pragma unroll
+          for (int b = 0; b < NB; ++b)
+            if (valid[b]) kvl[rowv[b]] = xs[b];
+          double djk[NB], dwv[NB], dxs[NB];
+          wave_tlm_group_djk<W>(xl, dxl, kvl, scr, kreg, dkreg, lane);
+          wave_row_sums<W>(scr, valid, djk, lane);
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            double v = dF[b];
+#pragma unroll
+            for (int j = 0; j < S - 1; ++j) {
+              const double c = j < s ? Tab::C[base + j] : 0.0;
+              if (c == 0.0) continue;
+              v = v + (c / dt) * dK[j][b];
+            }
+            dwv[b] = valid[b] ? v + djk[b] : 0.0;
+          }
+          int ld = ln;
+          asm volatile("" : "+v"(ld));
+          C_::forward(lu, rb, dwv, ld);
+          C_::backward(lu, rb, dwv, rdiag, dxs, ld);
+#pragma unroll
+          for (int j = 0; j < S; ++j)
+            if (j == s) {
+#pragma unroll
+              for (int b = 0; b < NB; ++b) dK[j][b] = dxs[b];
+            }
+        }
       }
       // error norm over all rows (fixed species add zeros)
       double term[NB];
@@ -479,7 +698,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
         term[b] = valid[b] ? (yerr / sci) * (yerr / sci) : 0.0;
       }
       double err_sq = 0.0;
-      if constexpr (MODE != 2) {
+      if constexpr (MODE != 2 && MODE != 4) {
         // every row's term summed over the wavefront: each lane adds its
         // banks, then a butterfly over the 64 lanes; partners add the same
         // two values, so every lane ends with the same sum
@@ -493,7 +712,7 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
       double err = sqrt(err_sq / W::N);
       err = fmax(err, 1.0e-10);
       const double factor_ = step_factor<Tab>(err);
-      if (MODE == 2 || err <= 1.0) {
+      if (MODE == 2 || MODE == 4 || err <= 1.0) {
         if constexpr (MODE == 1) {
           if (acc_steps >= adj->maxsteps) {
             status = ERR_CHECKPOINT_OVERFLOW;
@@ -518,6 +737,22 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
             first = false;
           }
           if (MODE != 3 || act[b]) x[b] += msum;
+        }
+        if constexpr (MODE == 4) {
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            double msum = 0.0;
+            bool first = true;
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+              const double m = Tab::M[j];
+              if (m == 0.0) continue;
+              const double tt = (m == 1.0) ? dK[j][b] : m * dK[j][b];
+              msum = first ? tt : msum + tt;
+              first = false;
+            }
+            dx[b] += msum;
+          }
         }
         t += dt;
         ++acc_steps;
@@ -544,6 +779,8 @@ __device__ __forceinline__ void wave_integrate_cell(const DeviceArgs& a, long ce
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     if (valid[b]) a.conc[cell + a.stride * (long)T_::VARPHYS[rowv[b]]] = x[b];
+    if constexpr (MODE == 4)
+      if (valid[b]) adj->dy[cell + a.stride * (long)T_::VARPHYS[rowv[b]]] = dx[b];
     fin = fin && (!valid[b] || isfinite(x[b]));
   }
   const bool all_fin = __ballot(!fin) == 0ull;

@@ -96,6 +96,8 @@ _SIGS["mkpp_amd_device_rate_constants"] = ([C.c_char_p, _vp, C.c_long, C.c_long,
 _SIGS["mkpp_amd_device_integrate_checkpointed"] = ([C.c_char_p, _vp, C.c_long, C.c_long, _vp, _vp, _vp, C.c_long, _vp,
                                                     C.c_long, C.c_double, C.c_int, C.c_double, _vp, _vp, _vp, _vp,
                                                     _vp, C.c_long, _vp, _vp, _vp, C.c_int, C.c_int, _vp], C.c_int)
+_SIGS["mkpp_amd_device_tangent_linear"] = ([C.c_char_p, _vp, C.c_long, C.c_long, _vp, _vp, _vp, C.c_long, _vp, C.c_long,
+                                            C.c_int, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp], C.c_int)
 _SIGS["mkpp_amd_device_adjoint"] = ([C.c_char_p, _vp, C.c_long, C.c_long, _vp, _vp, _vp, C.c_long, _vp, C.c_long,
                                      C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp], C.c_int)
 
@@ -510,3 +512,23 @@ def device_adjoint(mechanism: str, conc, method: str, temp, air_density, rate_pa
     if rc != MKPP_SUCCESS:
         raise MkppError(rc, "device_adjoint")
     return kbar
+
+
+def device_tangent_linear(mechanism: str, conc, method: str, temp, air_density, rate_parameters, k, hs, nsteps, dy,
+                          dk=None, status=None, stream=None):
+    """Tangent-linear replay of recorded steps (device tensors, in place):
+    conc (species, cells) y(0) -> y(T); dy (species, cells) dy(0) -> dy(T);
+    dk (groups, cells) or None the rate-constant direction; k as for the
+    adjoint (None: rate laws)."""
+    if hs is None or nsteps is None or dy is None:
+        raise ValueError(f"{mechanism}: hs, nsteps and dy are required")
+    ncell = validate(mechanism, conc, temp=temp, air_density=air_density, rate_parameters=rate_parameters, k=k,
+                     hs=hs, nsteps=nsteps, lam=dy, status=status, require_tolerances=False)
+    if dk is not None:
+        validate(mechanism, conc, k=dk, require_tolerances=False)
+    rc = lib().mkpp_amd_device_tangent_linear(
+        mechanism.encode(), _ptr(conc), ncell, ncell, _ptr(temp), _ptr(air_density), _ptr(rate_parameters),
+        ncell if rate_parameters is not None else 0, _ptr(k), ncell, METHODS[method], _ptr(hs), _ptr(nsteps),
+        int(hs.shape[1]), _ptr(dy), _ptr(dk), _ptr(status), _stream(stream))
+    if rc != MKPP_SUCCESS:
+        raise MkppError(rc, "device_tangent_linear")
