@@ -12,12 +12,15 @@ mechanism compiled into libmkpp_amd.so:
                                generation-time MKPP_DEFAULT_* options
   DIR/include/<mech>.hpp       mkpp::generated::<mech>::SolverKernels with
                                those options
+  DIR/include/mkpp_mod.f90     the Fortran module (the reference's procedure
+                               names, mkpp_mod.f90.j2:54-169)
   DIR/lib/libmkpp_<mech>.so    the unprefixed C ABI, starting every handle with
                                those options (linked to libmkpp_amd.so by a
                                relative run path)
 
 C callers compile with -I DIR/include and link DIR/lib/libmkpp_<mech>.so; C++
-callers of MechanismRegistry add -I <repo>/include so that mkpp_host/ is found.
+callers of MechanismRegistry add -I <repo>/include so that mkpp_host/ is found;
+Fortran callers compile DIR/include/mkpp_mod.f90 with their program.
 """
 
 from __future__ import annotations

@@ -13,6 +13,8 @@ This module writes both for mkpp_amd:
 * ``<mech>.hpp``: ``mkpp::generated::<mech>::SolverKernels<MemorySpace>``,
   whose ``integrate`` runs every cell of a (cells x species) view through the
   C ABI with the type's solver options;
+* ``<mech>/mkpp_mod.f90``: the Fortran module of the drop-in
+  (src/mkpp/templates/host_api/mkpp_mod.f90.j2:54-169);
 * ``mkpp_host/mechanisms.hpp``: the includes of every bundled mechanism's
   header that is on the include path, and the lookup of its generation-time
   solver options by name (what ``MechanismRegistry`` dispatches on).
@@ -317,6 +319,166 @@ def mechanisms_header(names: list) -> str:
     return "\n".join(L) + "\n"
 
 
+def fortran_module(p: dict, d: SolverDefaults) -> str:
+    """<mech>/mkpp_mod.f90: the Fortran module of the drop-in C ABI with the
+    reference's procedure names and argument meaning
+    (src/mkpp/templates/host_api/mkpp_mod.f90.j2:54-169): each procedure
+    returns its C return code in `status`; species and photolysis indices are
+    1-based; arrays are (cells, species) and (cells, parameters), which is the
+    C ABI's LayoutLeft order."""
+    m = _meta(p)
+    L = [
+        "!" + "=" * 78,
+        f"! mkpp_mod.f90 -- Fortran binding of the compiled mechanism {m['name']} (libmkpp_{m['name']}.so).",
+        "! Generated by mkpp_amd.gen.emit_host; do not edit.",
+        "!" + "=" * 78,
+        "module mkpp_mod",
+        "  use, intrinsic :: iso_c_binding",
+        "  implicit none",
+        "  private",
+        "",
+        "  integer(c_int), parameter, public :: MKPP_SUCCESS = 0_c_int",
+        "  integer(c_int), parameter, public :: MKPP_ERROR_NULL_PTR = -101_c_int",
+        "  integer(c_int), parameter, public :: MKPP_ERROR_INVALID_HDL = -102_c_int",
+        "  integer(c_int), parameter, public :: MKPP_ERROR_INVALID_ARG = -103_c_int",
+        "  integer(c_int), parameter, public :: MKPP_ERROR_INVALID_STE = -104_c_int",
+        "  integer(c_int), parameter, public :: MKPP_ERROR_CONVERGENCE = -105_c_int",
+        "  integer(c_int), parameter, public :: MKPP_ERROR_NOT_FOUND = -106_c_int",
+        "",
+        f"  integer, parameter, public :: MKPP_NUM_RATE_PARAMETERS = {m['P']}",
+        f"  integer, parameter, public :: MKPP_NUM_SPECIES = {m['n']}",
+        f"  integer, parameter, public :: MKPP_NUM_REACTIONS = {m['reactions']}",
+        f"  integer, parameter, public :: MKPP_NUM_PHOTOLYSIS = {m['photolysis']}",
+        "  ! generation-time solver options of this drop-in (method id 0 ROS-2 ... 4 RODAS-4)",
+        f"  integer, parameter, public :: MKPP_DEFAULT_METHOD = {METHOD_IDS[d.method]}",
+        "",
+        "  type, public :: mkpp_handle_t",
+        "    type(c_ptr) :: ptr = c_null_ptr",
+        "  end type mkpp_handle_t",
+        "",
+    ]
+    procs = ["create_handle", "destroy_handle", "initialize", "get_species_count", "get_reaction_count",
+             "get_photolysis_count", "get_species_index", "get_photolysis_index", "set_state_ptrs",
+             "set_rate_parameter_ptrs", "set_photolysis_ptrs", "set_cloud_water_ptr", "set_aerosol_ptrs", "integrate",
+             "evaluate_rates", "evaluate_jacobian", "get_last_error"]
+    for pr in procs:
+        L.append(f"  public :: mkpp_{pr}")
+    # C interfaces: (name, [(arg, declaration)])
+    h = ("handle", "type(c_ptr), value :: handle")
+    cints = {
+        "create_handle": [("handle_out", "type(c_ptr), intent(out) :: handle_out"),
+                          ("num_cells", "integer(c_size_t), value :: num_cells")],
+        "destroy_handle": [h],
+        "initialize": [h],
+        "get_species_count": [h, ("count_out", "integer(c_size_t), intent(out) :: count_out")],
+        "get_reaction_count": [h, ("count_out", "integer(c_size_t), intent(out) :: count_out")],
+        "get_photolysis_count": [h, ("count_out", "integer(c_size_t), intent(out) :: count_out")],
+        "get_species_index": [h, ("name", "character(kind=c_char), intent(in) :: name(*)"),
+                              ("index_out", "integer(c_int), intent(out) :: index_out")],
+        "get_photolysis_index": [h, ("name", "character(kind=c_char), intent(in) :: name(*)"),
+                                 ("index_out", "integer(c_int), intent(out) :: index_out")],
+        "set_state_ptrs": [h] + [(a, f"type(c_ptr), value :: {a}") for a in ("conc", "temp", "pres", "rho")],
+        "set_rate_parameter_ptrs": [h, ("values", "type(c_ptr), value :: values"),
+                                    ("count", "integer(c_size_t), value :: count")],
+        "set_photolysis_ptrs": [h, ("values", "type(c_ptr), value :: values")],
+        "set_cloud_water_ptr": [h, ("values", "type(c_ptr), value :: values")],
+        "set_aerosol_ptrs": [h, ("rh", "type(c_ptr), value :: rh"), ("sa", "type(c_ptr), value :: sa")],
+        "integrate": [h, ("dt", "real(c_double), value :: dt")],
+        "evaluate_rates": [h, ("out", "type(c_ptr), value :: out")],
+        "evaluate_jacobian": [h, ("out", "type(c_ptr), value :: out")],
+        "get_last_error": [h, ("buffer", "character(kind=c_char), intent(out) :: buffer(*)"),
+                           ("buffer_len", "integer(c_size_t), value :: buffer_len")],
+    }
+    L += ["", "  interface"]
+    for pr in procs:
+        args = cints[pr]
+        L.append(f"    integer(c_int) function c_{pr}({', '.join(a for a, _ in args)}) bind(C, name=\"mkpp_{pr}\")")
+        L.append("      import :: c_ptr, c_int, c_size_t, c_double, c_char")
+        for _, decl in args:
+            L.append(f"      {decl}")
+        L.append(f"    end function c_{pr}")
+    L += ["  end interface", "", "contains", ""]
+    hd = "    type(mkpp_handle_t), intent(in) :: handle"
+    st = "    integer, intent(out) :: status"
+    body = {
+        "create_handle": (["handle", "num_cells", "status"],
+                          ["    type(mkpp_handle_t), intent(out) :: handle", "    integer, intent(in) :: num_cells", st],
+                          ["    status = int(c_create_handle(handle%ptr, int(num_cells, c_size_t)))"]),
+        "destroy_handle": (["handle", "status"], ["    type(mkpp_handle_t), intent(inout) :: handle", st],
+                           ["    status = int(c_destroy_handle(handle%ptr))", "    handle%ptr = c_null_ptr"]),
+        "initialize": (["handle", "status"], [hd, st], ["    status = int(c_initialize(handle%ptr))"]),
+    }
+    for cnt in ("species", "reaction", "photolysis"):
+        body[f"get_{cnt}_count"] = (["handle", "count_out", "status"],
+                                    [hd, "    integer, intent(out) :: count_out", st, "    integer(c_size_t) :: n"],
+                                    [f"    status = int(c_get_{cnt}_count(handle%ptr, n))", "    count_out = int(n)"])
+    for what in ("species", "photolysis"):
+        body[f"get_{what}_index"] = (
+            ["handle", "name", "index_out", "status"],
+            [hd, "    character(len=*), intent(in) :: name", "    integer, intent(out) :: index_out", st,
+             "    integer(c_int) :: i0"],
+            [f"    status = int(c_get_{what}_index(handle%ptr, trim(name) // c_null_char, i0))",
+             "    index_out = -1",
+             "    if (status == MKPP_SUCCESS) index_out = int(i0) + 1  ! 1-based"])
+    body["set_state_ptrs"] = (
+        ["handle", "conc", "temp", "pres", "rho", "status"],
+        [hd, "    real(c_double), target, intent(inout) :: conc(:, :)  ! (cells, species)",
+         "    real(c_double), target, intent(in) :: temp(:), pres(:), rho(:)", st],
+        ["    if (.not. (is_contiguous(conc) .and. is_contiguous(temp) .and. is_contiguous(pres) .and. &",
+         "               is_contiguous(rho))) then",
+         "      status = MKPP_ERROR_INVALID_ARG",
+         "      return",
+         "    end if",
+         "    status = int(c_set_state_ptrs(handle%ptr, c_loc(conc), c_loc(temp), c_loc(pres), c_loc(rho)))"])
+    body["set_rate_parameter_ptrs"] = (
+        ["handle", "values", "status"],
+        [hd, "    real(c_double), target, intent(in) :: values(:, :)  ! (cells, MKPP_NUM_RATE_PARAMETERS)", st],
+        ["    if (.not. is_contiguous(values) .or. size(values, 2) /= MKPP_NUM_RATE_PARAMETERS) then",
+         "      status = MKPP_ERROR_INVALID_ARG",
+         "      return",
+         "    end if",
+         "    status = int(c_set_rate_parameter_ptrs(handle%ptr, c_loc(values), int(size(values, 2), c_size_t)))"])
+    body["set_photolysis_ptrs"] = (
+        ["handle", "values", "status"],
+        [hd, "    real(c_double), target, intent(in) :: values(:, :)  ! (cells, MKPP_NUM_PHOTOLYSIS)", st],
+        ["    if (.not. is_contiguous(values)) then",
+         "      status = MKPP_ERROR_INVALID_ARG",
+         "      return",
+         "    end if",
+         "    status = int(c_set_photolysis_ptrs(handle%ptr, c_loc(values)))"])
+    body["set_cloud_water_ptr"] = (
+        ["handle", "values", "status"], [hd, "    real(c_double), target, intent(in) :: values(:)", st],
+        ["    status = int(c_set_cloud_water_ptr(handle%ptr, c_loc(values)))"])
+    body["set_aerosol_ptrs"] = (
+        ["handle", "rh", "sa", "status"], [hd, "    real(c_double), target, intent(in) :: rh(:), sa(:)", st],
+        ["    status = int(c_set_aerosol_ptrs(handle%ptr, c_loc(rh), c_loc(sa)))"])
+    body["integrate"] = (["handle", "dt", "status"], [hd, "    real(c_double), intent(in) :: dt", st],
+                         ["    status = int(c_integrate(handle%ptr, dt))"])
+    body["evaluate_rates"] = (
+        ["handle", "rates", "status"], [hd, "    real(c_double), target, intent(out) :: rates(:, :)  ! (cells, species)", st],
+        ["    status = int(c_evaluate_rates(handle%ptr, c_loc(rates)))"])
+    body["evaluate_jacobian"] = (
+        ["handle", "jac", "status"],
+        [hd, "    real(c_double), target, intent(out) :: jac(:, :)  ! (cells, species * species), column j*N + i", st],
+        ["    status = int(c_evaluate_jacobian(handle%ptr, c_loc(jac)))"])
+    body["get_last_error"] = (
+        ["handle", "message"],
+        [hd, "    character(len=*), intent(out) :: message", "    character(kind=c_char) :: buf(512)",
+         "    integer(c_int) :: rc", "    integer :: i"],
+        ["    buf = c_null_char", "    rc = c_get_last_error(handle%ptr, buf, 512_c_size_t)", "    message = ''",
+         "    do i = 1, min(len(message), 512)", "      if (buf(i) == c_null_char) exit",
+         "      message(i:i) = buf(i)", "    end do"])
+    for pr in procs:
+        args, decls, stmts = body[pr]
+        L.append(f"  subroutine mkpp_{pr}({', '.join(args)})")
+        L += decls
+        L += stmts
+        L.append(f"  end subroutine mkpp_{pr}")
+        L.append("")
+    L.append("end module mkpp_mod")
+    return "\n".join(L) + "\n"
+
+
 def write_mechanism(p: dict, d: SolverDefaults, include_dir: str, c_dir: str | None = None) -> list:
     """Writes <include_dir>/<mech>.hpp and <c_dir or include_dir/<mech>>/mkpp.h."""
     name = p["name"]
@@ -325,7 +487,8 @@ def write_mechanism(p: dict, d: SolverDefaults, include_dir: str, c_dir: str | N
     os.makedirs(c_dir, exist_ok=True)
     out = []
     for path, text in ((os.path.join(include_dir, f"{name}.hpp"), cpp_header(p, d)),
-                       (os.path.join(c_dir, "mkpp.h"), c_header(p, d))):
+                       (os.path.join(c_dir, "mkpp.h"), c_header(p, d)),
+                       (os.path.join(c_dir, "mkpp_mod.f90"), fortran_module(p, d))):
         with open(path, "w", encoding="utf-8") as f:
             f.write(text)
         out.append(path)

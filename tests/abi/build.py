@@ -8,6 +8,10 @@ but travels with the tree):
   bin/c_abi_saprc99       c_abi_main.c, gcc, against that drop-in's mkpp.h
   bin/registry_saprc99    registry_main.cpp, g++, against include/mkpp_host
                           and that drop-in's saprc99.hpp
+  bin/fortran_saprc99     f_abi_main.f90, flang, against that drop-in's
+                          mkpp_mod.f90
+  bin/fortran_device      f_device_main.f90, flang, against
+                          include/mkpp_amd_mod.f90 and libmkpp_amd.so
 """
 
 from __future__ import annotations
@@ -20,6 +24,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 BIN = os.path.join(HERE, "bin")
 DROPIN = os.path.join(BIN, "saprc99_rodas3")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+FLANG = os.path.join(ROCM, "lib", "llvm", "bin", "flang")
 
 
 def _run(cmd):
@@ -45,7 +50,18 @@ def build() -> dict:
           "-L", libdir, "-lmkpp_amd", "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
           f"-Wl,-rpath,$ORIGIN/{os.path.relpath(libdir, BIN)}", "-Wl,-rpath," + os.path.join(ROCM, "lib"),
           "-o", cpp_exe])
-    info.update(c_program=c_exe, cpp_program=cpp_exe)
+    f_exe = os.path.join(BIN, "fortran_saprc99")
+    mod_dir = os.path.join(BIN, "fmod")
+    os.makedirs(mod_dir, exist_ok=True)
+    _run([FLANG, "-O2", "-J", mod_dir, os.path.join(DROPIN, "include", "mkpp_mod.f90"),
+          os.path.join(HERE, "f_abi_main.f90"), "-L", os.path.join(DROPIN, "lib"), "-lmkpp_saprc99",
+          "-Wl,-rpath,$ORIGIN/saprc99_rodas3/lib", "-o", f_exe])
+    fd_exe = os.path.join(BIN, "fortran_device")
+    _run([FLANG, "-O2", "-J", mod_dir, os.path.join(ROOT, "include", "mkpp_amd_mod.f90"),
+          os.path.join(HERE, "f_device_main.f90"), "-L", libdir, "-lmkpp_amd", "-L", os.path.join(ROCM, "lib"),
+          "-lamdhip64", f"-Wl,-rpath,$ORIGIN/{os.path.relpath(libdir, BIN)}", "-Wl,-rpath," + os.path.join(ROCM, "lib"),
+          "-o", fd_exe])
+    info.update(c_program=c_exe, cpp_program=cpp_exe, fortran_program=f_exe, fortran_device_program=fd_exe)
     return info
 
 

@@ -8,6 +8,10 @@
   and a program printing its SolverKernels constants agrees with the
   registry and the plan;
 * MechanismRegistry's name queries and its unknown-name error run on the host;
+* every generated mkpp_mod.f90 compiles with flang, binds exactly the
+  reference module's procedures (mkpp_mod.f90.j2:54-169) and carries the
+  registry's counts; include/mkpp_amd_mod.f90 compiles and binds every
+  mkpp_amd_* function mkpp_amd.h declares;
 * drop-ins start their handles with the options they were generated with:
   the in-tree ones with the reference's ROS-3 / 1e-3 / 1e-6 / 1e-6, the one
   mkpp_amd.dropin built from tests/abi/env_rodas3.yaml with Rodas3 / 1e3 /
@@ -116,6 +120,49 @@ int main() {
         assert (int(ns), int(nr), int(env)) == (info["num_species"], info["num_reactions"],
                                                 info["supports_runtime_environment"])
     assert "ok 1" in out
+
+
+FLANG = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "llvm", "bin", "flang")
+REFERENCE_F90_PROCS = sorted(
+    "mkpp_" + p for p in ("create_handle", "destroy_handle", "initialize", "get_species_count", "get_reaction_count",
+                          "get_photolysis_count", "get_species_index", "get_photolysis_index", "set_state_ptrs",
+                          "set_rate_parameter_ptrs", "set_photolysis_ptrs", "set_cloud_water_ptr", "set_aerosol_ptrs",
+                          "integrate", "evaluate_rates", "evaluate_jacobian", "get_last_error"))
+
+
+def _f90_params(path):
+    return {k: int(v) for k, v in re.findall(r"integer, parameter, public :: (MKPP_\w+) = (-?\d+)", open(path).read())}
+
+
+@pytest.mark.parametrize("mech", BUNDLED)
+def test_generated_fortran_module(mech, tmp_path):
+    path = os.path.join(MECH_INC, mech, "mkpp_mod.f90")
+    text = open(path).read()
+    assert sorted(set(re.findall(r"^  subroutine (mkpp_\w+)\(", text, flags=re.M))) == REFERENCE_F90_PROCS
+    assert sorted(set(re.findall(r'bind\(C, name="(mkpp_\w+)"\)', text))) == REFERENCE_F90_PROCS
+    prm = _f90_params(path)
+    info = native.mechanism_info(mech)
+    assert prm["MKPP_NUM_SPECIES"] == info["num_species"]
+    assert prm["MKPP_NUM_REACTIONS"] == info["num_reactions"]
+    assert prm["MKPP_NUM_RATE_PARAMETERS"] == info["num_rate_parameters"]
+    assert prm["MKPP_DEFAULT_METHOD"] == 1
+    r = subprocess.run([FLANG, "-c", "-J", str(tmp_path), path, "-o", str(tmp_path / "m.o")], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_device_fortran_module_binds_every_entry_point(tmp_path):
+    f90 = os.path.join(INC, "mkpp_amd_mod.f90")
+    bound = sorted(set(re.findall(r'bind\(C, name="(mkpp_amd_\w+)"\)', open(f90).read())))
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(INC, "mkpp_amd.h")).read(), flags=re.S)
+    declared = sorted(set(re.findall(r"\b(mkpp_amd_\w+)\s*\(", text)))
+    assert bound == declared
+    r = subprocess.run([FLANG, "-c", "-J", str(tmp_path), f90, "-o", str(tmp_path / "m.o")], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr
+    lib = native.lib()
+    for name in bound:
+        assert hasattr(lib, name), name
 
 
 def _options(dropin_path, n):
