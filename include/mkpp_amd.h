@@ -121,6 +121,19 @@ int mkpp_amd_device_integrate_reduced(const char* mechanism, double* conc, long 
                                       long parameter_stride, double dt, int steps, int method,
                                       double importance_threshold, const double* atol, const double* rtol,
                                       int* status, int* accepted, int* attempted, long max_attempts, void* stream);
+/* Validation backend: the same integration over the dense unrolled LU and
+ * solves the reference compiles under MKPP_USE_UNROLLED_REFERENCE
+ * (src/mkpp/templates/solver_variants/integrate.j2:136-141, 255-259;
+ * device_impl/factorize.hpp.j2:13-18, device_impl/solve.hpp.j2:12-21): one
+ * cell per lane, N x N matrices in the solver permutation.  Lane-per-cell
+ * mechanisms only (MKPP_ERROR_INVALID_ARGUMENT otherwise); slow by design.
+ * Arguments and results as mkpp_amd_device_integrate, whose results it equals
+ * bit for bit. */
+int mkpp_amd_device_integrate_unrolled(const char* mechanism, double* conc, long num_cells, long stride,
+                                       const double* temp, const double* air_density, const double* rate_parameters,
+                                       long parameter_stride, double dt, int steps, int method,
+                                       double initial_step_seed, const double* atol, const double* rtol, int* status,
+                                       int* accepted, int* attempted, long max_attempts, void* stream);
 /* what: 0 = rates (out[cell + out_stride*species]),
  *       1 = dense Jacobian (out[cell + out_stride*(i + num_species*j)]). */
 int mkpp_amd_device_evaluate(const char* mechanism, int what, const double* conc, long num_cells, long stride,

@@ -11,7 +11,7 @@
 !   * the handle API with the mechanism chosen at creation, and its solver
 !     option, step and per-cell status extensions;
 !   * the stream-ordered device API on device pointers (integrate, reduced
-!     integrate, rates/Jacobian), the persistent multi-mechanism batch (with
+!     integrate, the unrolled validation backend, rates/Jacobian), the persistent multi-mechanism batch (with
 !     and without a workspace), registry-style execute, and the
 !     differentiation entry points (checkpointed forward, adjoint,
 !     tangent-linear).
@@ -108,6 +108,7 @@ module mkpp_amd_mod
   public :: mkpp_amd_set_solver_options, mkpp_amd_get_solver_options, mkpp_amd_set_max_attempts
   public :: mkpp_amd_integrate_steps, mkpp_amd_get_cell_status
   public :: mkpp_amd_device_integrate, mkpp_amd_device_integrate_reduced, mkpp_amd_device_evaluate
+  public :: mkpp_amd_device_integrate_unrolled
   public :: mkpp_amd_device_integrate_batch, mkpp_amd_device_integrate_batch_static, mkpp_amd_batch_capacity
   public :: mkpp_amd_execute
   public :: mkpp_amd_diff_info, mkpp_amd_device_rate_constants, mkpp_amd_device_integrate_checkpointed
@@ -298,6 +299,25 @@ module mkpp_amd_mod
       real(c_double), value :: dt
       integer(c_int), value :: steps, method
       real(c_double), value :: importance_threshold
+      type(c_ptr), value :: atol, rtol, status, accepted, attempted
+      integer(c_long), value :: max_attempts
+      type(c_ptr), value :: stream
+    end function
+    ! validation backend over the dense unrolled LU (lane-per-cell mechanisms)
+    integer(c_int) function mkpp_amd_device_integrate_unrolled(mechanism, conc, num_cells, stride, temp, &
+                                                               air_density, rate_parameters, parameter_stride, dt, &
+                                                               steps, method, initial_step_seed, atol, rtol, &
+                                                               status, accepted, attempted, max_attempts, stream) &
+        bind(C, name="mkpp_amd_device_integrate_unrolled")
+      import :: c_int, c_ptr, c_char, c_long, c_double
+      character(kind=c_char), intent(in) :: mechanism(*)
+      type(c_ptr), value :: conc
+      integer(c_long), value :: num_cells, stride
+      type(c_ptr), value :: temp, air_density, rate_parameters
+      integer(c_long), value :: parameter_stride
+      real(c_double), value :: dt
+      integer(c_int), value :: steps, method
+      real(c_double), value :: initial_step_seed
       type(c_ptr), value :: atol, rtol, status, accepted, attempted
       integer(c_long), value :: max_attempts
       type(c_ptr), value :: stream

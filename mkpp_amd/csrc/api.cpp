@@ -612,6 +612,40 @@ int mkpp_amd_device_integrate(const char* mechanism, double* conc, long num_cell
   return o->integrate(a, (hipStream_t)stream) == 0 ? MKPP_SUCCESS : MKPP_ERROR_INVALID_STATE;
 }
 
+int mkpp_amd_device_integrate_unrolled(const char* mechanism, double* conc, long num_cells, long stride,
+                                       const double* temp, const double* air_density, const double* rate_parameters,
+                                       long parameter_stride, double dt, int steps, int method,
+                                       double initial_step_seed, const double* atol, const double* rtol, int* status,
+                                       int* accepted, int* attempted, long max_attempts, void* stream) {
+  const MechOps* o = mkpp::find_mechanism(mechanism);
+  if (!mechanism || !conc || !atol || !rtol) return MKPP_ERROR_NULL_POINTER;
+  if (!o) return MKPP_ERROR_NOT_FOUND;
+  if (num_cells < 0 || stride < num_cells || !std::isfinite(dt) || dt <= 0.0 || steps < 1 ||
+      method < mkpp::ROS2 || method > mkpp::RODAS4 || !(initial_step_seed > 0.0))
+    return MKPP_ERROR_INVALID_ARGUMENT;
+  if (rate_parameters && parameter_stride < num_cells) return MKPP_ERROR_INVALID_ARGUMENT;
+  if (!o->integrate_unrolled) return MKPP_ERROR_INVALID_ARGUMENT;
+  DeviceArgs a{};
+  a.conc = conc;
+  a.ncells = num_cells;
+  a.stride = stride;
+  a.temp = temp;
+  a.air_density = air_density;
+  a.jvals = rate_parameters;
+  a.jstride = parameter_stride;
+  a.dt = dt;
+  a.steps = steps;
+  a.method = method;
+  a.seed = initial_step_seed;
+  a.atol = atol;
+  a.rtol = rtol;
+  a.status = status;
+  a.accepted = accepted;
+  a.attempted = attempted;
+  a.max_attempts = max_attempts;
+  return o->integrate_unrolled(a, (hipStream_t)stream) == 0 ? MKPP_SUCCESS : MKPP_ERROR_INVALID_STATE;
+}
+
 int mkpp_amd_device_integrate_reduced(const char* mechanism, double* conc, long num_cells, long stride,
                                       const double* temp, const double* air_density, const double* rate_parameters,
                                       long parameter_stride, double dt, int steps, int method,

@@ -13,8 +13,8 @@ __device__ const uint16_t LOFS[2] = {0, 0};
 __device__ const uint8_t LROW[1] = {0};
 __device__ const uint16_t UOFS[2] = {0, 0};
 __device__ const uint8_t UCOL[1] = {0};
-__device__ const uint64_t FMASK[1] = {0};
-__device__ const uint64_t BMASK[1] = {0};
+__device__ const uint64_t FMASK[1] = {0ull};
+__device__ const uint64_t BMASK[1] = {0ull};
 __device__ const uint8_t LAW[1] = {0};
 __device__ const int16_t PHOTO[1] = {0};
 __device__ const double PARAMS[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};

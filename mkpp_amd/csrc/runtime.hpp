@@ -31,6 +31,9 @@ struct MechOps {
   int num_variable;  // variable species (checkpoint row length)
   // integrate_with_reduction (a.importance_threshold; tolerances by permuted position)
   int (*integrate_reduced)(const DeviceArgs& a, hipStream_t s);
+  // validation backend: integrate over the dense unrolled LU (lane mechanisms
+  // only, null otherwise)
+  int (*integrate_unrolled)(const DeviceArgs& a, hipStream_t s);
 };
 
 const MechOps* find_mechanism(const char* name);
@@ -87,7 +90,8 @@ int launch_eval_jacobian(const EvalArgs& a, hipStream_t s) {
 }  // namespace mkpp
 
 // Defines mkpp_ops_<name>() for a lane-per-cell mechanism whose generated
-// header declares mkpp::gen::<name>::Mech.
+// headers declare mkpp::gen::<name>::Mech and mkpp::gen::<name>_unrolled::Unrolled
+// (the unit includes unrolled.hpp).
 #define MKPP_LANE_MECHANISM(NAME, F0, F1, F2, F3, F4)                                                        \
   namespace mkpp {                                                                                         \
   const MechOps* mkpp_ops_##NAME() {                                                                       \
@@ -107,7 +111,8 @@ int launch_eval_jacobian(const EvalArgs& a, hipStream_t s) {
                                 nullptr,                                                                   \
                                 M::NG,                                                                     \
                                 M::NV,                                                                     \
-                                &launch_lane_reduced<M>};                                                  \
+                                &launch_lane_reduced<M>,                                                   \
+                                &launch_unrolled_integrate<M, gen::NAME##_unrolled::Unrolled>};            \
     return &ops;                                                                                           \
   }                                                                                                        \
   }

@@ -40,7 +40,10 @@ def main(argv=None):
         if not args.no_hip and name in BUNDLED:
             from . import emit_hip, emit_wave
 
+            from . import emit_unrolled
+
             outdir = os.path.join(root, "mkpp_amd", "csrc", "gen")
+            emit_unrolled.write(p, outdir)
             nv = len(p["species"]) - sum(p["fixed"])
             if nv <= emit_hip.LANE_MAX_VARIABLE_SPECIES and name not in FORCE_WAVE:
                 emit_hip.write(mech, p, outdir)

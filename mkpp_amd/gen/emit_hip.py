@@ -419,7 +419,9 @@ def emit_unit(p: dict) -> str:
     return (
         f"// Generated by mkpp_amd.gen.emit_hip: registers {name} (one cell per lane).\n"
         f'#include "{name}.hpp"\n'
-        '#include "../runtime.hpp"\n\n'
+        '#include "../runtime.hpp"\n'
+        '#include "../unrolled.hpp"\n'
+        f'#include "{name}_unrolled.hpp"\n\n'
         f"MKPP_LANE_MECHANISM({name}, {flops})\n"
     )
 

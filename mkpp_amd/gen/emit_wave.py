@@ -330,7 +330,8 @@ def _arr(ctype: str, name: str, vals) -> str:
     vals = list(vals)
     if not vals:
         vals = [0]
-    body = ", ".join(repr(float(v)) if ctype == "double" else str(int(v)) for v in vals)
+    sfx = "ull" if ctype == "uint64_t" else ""
+    body = ", ".join(repr(float(v)) if ctype == "double" else f"{int(v)}{sfx}" for v in vals)
     return f"__device__ const {ctype} {name}[{len(vals)}] = {{{body}}};"
 
 

@@ -65,6 +65,9 @@ _SIGS = {
     "mkpp_amd_get_cell_status": ([_vp, _ip, _ip, _ip], C.c_int),
     "mkpp_amd_device_integrate": ([C.c_char_p, _vp, C.c_long, C.c_long, _vp, _vp, _vp, C.c_long, C.c_double, C.c_int,
                                    C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, C.c_long, _vp], C.c_int),
+    "mkpp_amd_device_integrate_unrolled": ([C.c_char_p, _vp, C.c_long, C.c_long, _vp, _vp, _vp, C.c_long, C.c_double,
+                                            C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, C.c_long, _vp],
+                                           C.c_int),
     "mkpp_amd_device_evaluate": ([C.c_char_p, C.c_int, _vp, C.c_long, C.c_long, _vp, _vp, _vp, C.c_long, _vp, C.c_long,
                                   _vp], C.c_int),
     "mkpp_amd_device_integrate_reduced": ([C.c_char_p, _vp, C.c_long, C.c_long, _vp, _vp, _vp, C.c_long, C.c_double,
@@ -344,6 +347,26 @@ def device_integrate(mechanism: str, conc, dt: float, steps: int = 1, method: st
         C.c_void_p(s.cuda_stream))
     if rc != MKPP_SUCCESS:
         raise MkppError(rc, "device_integrate")
+
+
+def device_integrate_unrolled(mechanism: str, conc, dt: float, steps: int = 1, method: str = "rodas3", temp=None,
+                              air_density=None, rate_parameters=None, atol=None, rtol=None, seed: float = 1.0e-6,
+                              status=None, accepted=None, attempted=None, max_attempts: int = 100000, stream=None):
+    """device_integrate through the validation backend: dense unrolled LU and
+    solves in the solver permutation (the reference's
+    MKPP_USE_UNROLLED_REFERENCE build).  Lane-per-cell mechanisms only."""
+    import torch
+
+    ncell = validate(mechanism, conc, temp=temp, air_density=air_density, rate_parameters=rate_parameters,
+                     atol=atol, rtol=rtol, status=status, accepted=accepted, attempted=attempted)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    rc = lib().mkpp_amd_device_integrate_unrolled(
+        mechanism.encode(), _ptr(conc), ncell, ncell, _ptr(temp), _ptr(air_density), _ptr(rate_parameters),
+        ncell if rate_parameters is not None else 0, float(dt), int(steps), METHODS[method], float(seed),
+        _ptr(atol), _ptr(rtol), _ptr(status), _ptr(accepted), _ptr(attempted), int(max_attempts),
+        C.c_void_p(s.cuda_stream))
+    if rc != MKPP_SUCCESS:
+        raise MkppError(rc, "device_integrate_unrolled")
 
 
 def device_integrate_reduced(mechanism: str, conc, dt: float, steps: int = 1, method: str = "rodas3", temp=None,
