@@ -579,8 +579,7 @@ __global__ void __launch_bounds__(64) wave_rate_constants_kernel(DeviceArgs a, d
 }
 
 // mode 0: rate constants only; 1: checkpointed forward; 2: replay; 3: adjoint;
-// 4: tangent-linear replay.  Only Rodas3 and Ros4 are instantiated for
-// differentiation.
+// 4: tangent-linear replay; every tableau.
 template <class W>
 int launch_wave_diff(int mode, const DeviceArgs& a, const AdjArgs& g, hipStream_t s) {
   if (a.ncells <= 0) return 0;
@@ -594,13 +593,15 @@ int launch_wave_diff(int mode, const DeviceArgs& a, const AdjArgs& g, hipStream_
   else if (mode == 2) wave_record_kernel<W, TAB, 2><<<grid, 64, fwd_lds, s>>>(a, g, base);               \
   else if (mode == 4) wave_tlm_kernel<W, TAB><<<grid, 64, sizeof(double) * wave_tlm_lds_doubles<W>(), s>>>(a, g, base); \
   else wave_adjoint_kernel<W, TAB><<<grid, 64, sizeof(double) * adjoint_lds_doubles<W, TAB>(), s>>>(a, g, base);
-  if (a.method != RODAS3 && a.method != ROS4) return -1;
+  if (a.method < ROS2 || a.method > RODAS4) return -1;
   for (long base = 0; base < a.ncells; base += WAVE_CHUNK_CELLS) {
     const unsigned grid = wave_chunk_grid(a.ncells, base);
-    if (a.method == RODAS3) {
-      MKPP_DIFF_CASE(TabRodas3)
-    } else {
-      MKPP_DIFF_CASE(TabRos4)
+    switch (a.method) {
+      case ROS2: MKPP_DIFF_CASE(TabRos2) break;
+      case ROS3: MKPP_DIFF_CASE(TabRos3) break;
+      case ROS4: MKPP_DIFF_CASE(TabRos4) break;
+      case RODAS3: MKPP_DIFF_CASE(TabRodas3) break;
+      default: MKPP_DIFF_CASE(TabRodas4) break;
     }
   }
 #undef MKPP_DIFF_CASE

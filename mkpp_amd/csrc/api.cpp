@@ -942,7 +942,7 @@ int mkpp_amd_device_integrate_checkpointed(const char* mechanism, double* conc, 
   if (!conc || !atol || !rtol || !ckpt || !hs || !nsteps) return MKPP_ERROR_NULL_POINTER;
   if (num_cells < 0 || stride < num_cells || (rate_parameters && parameter_stride < num_cells) ||
       (k && k_stride < num_cells) || !std::isfinite(dt) || dt <= 0.0 || !(initial_step_seed > 0.0) ||
-      max_steps < 1 || (method != mkpp::RODAS3 && method != mkpp::ROS4) || (replay != 0 && replay != 1))
+      max_steps < 1 || (method < mkpp::ROS2 || method > mkpp::RODAS4) || (replay != 0 && replay != 1))
     return MKPP_ERROR_INVALID_ARGUMENT;
   DeviceArgs a = diff_args(conc, num_cells, stride, temp, air_density, rate_parameters, parameter_stride);
   a.dt = dt;
@@ -974,7 +974,7 @@ int mkpp_amd_device_adjoint(const char* mechanism, const double* conc, long num_
   if (!o) return rc;
   if (!conc || !ckpt || !hs || !nsteps || !lambda || !kbar) return MKPP_ERROR_NULL_POINTER;
   if (num_cells < 0 || stride < num_cells || (rate_parameters && parameter_stride < num_cells) ||
-      k_stride < num_cells || max_steps < 1 || (method != mkpp::RODAS3 && method != mkpp::ROS4))
+      k_stride < num_cells || max_steps < 1 || (method < mkpp::ROS2 || method > mkpp::RODAS4))
     return MKPP_ERROR_INVALID_ARGUMENT;
   DeviceArgs a = diff_args(conc, num_cells, stride, temp, air_density, rate_parameters, parameter_stride);
   a.method = method;
@@ -1000,7 +1000,7 @@ int mkpp_amd_device_tangent_linear(const char* mechanism, double* conc, long num
   if (!o) return rc;
   if (!conc || !hs || !nsteps || !dy) return MKPP_ERROR_NULL_POINTER;
   if (num_cells < 0 || stride < num_cells || (rate_parameters && parameter_stride < num_cells) ||
-      ((k || dk) && k_stride < num_cells) || max_steps < 1 || (method != mkpp::RODAS3 && method != mkpp::ROS4))
+      ((k || dk) && k_stride < num_cells) || max_steps < 1 || (method < mkpp::ROS2 || method > mkpp::RODAS4))
     return MKPP_ERROR_INVALID_ARGUMENT;
   DeviceArgs a = diff_args(conc, num_cells, stride, temp, air_density, rate_parameters, parameter_stride);
   a.method = method;
