@@ -85,7 +85,7 @@ def test_controller_kat_counts_and_state(mech, method):
         assert (y == y[0]).all()
 
 
-@pytest.mark.parametrize("method", ["rodas3", "ros4"])
+@pytest.mark.parametrize("method", ["ros2", "ros3", "ros4", "rodas3", "rodas4"])
 def test_controller_kat_accepted_dt_sequence_wave(method):
     """The wave engine's recorded accepted steps are the closed-form sequence."""
     dev = torch.device("cuda:0")
