@@ -236,7 +236,10 @@ typedef struct mkpp_amd_execution_params {
 int mkpp_amd_execute(mkpp_handle_t handle, double* conc, long num_cells, long stride,
                      const mkpp_amd_execution_params* params, void* stream);
 
-/* ---- 5. differentiation (wave-per-cell mechanisms, every method) ----
+/* ---- 5. differentiation (every mechanism and method) ----
+ * The lane-per-cell mechanisms run these entry points on the wave engine's
+ * tables of the same mechanism: their checkpointed forward equals
+ * mkpp_amd_device_integrate within rounding, not bit for bit.
  * Replaces the reference's frozen-Jacobian checkpoint / adjoint templates
  * (solver_variants/integrate_fwd_checkpoint.j2:37-295, integrate_adj.j2:50-156)
  * with the exact adjoint of the discrete Rosenbrock map over the accepted step

@@ -380,7 +380,7 @@ module mkpp_amd_mod
       type(c_ptr), value :: stream
     end function
 
-    ! ---- differentiation (wave-per-cell mechanisms; every method) ----
+    ! ---- differentiation (every mechanism and method) ----
     integer(c_int) function mkpp_amd_diff_info(mechanism, num_groups, num_variable) bind(C, name="mkpp_amd_diff_info")
       import :: c_int, c_char
       character(kind=c_char), intent(in) :: mechanism(*)

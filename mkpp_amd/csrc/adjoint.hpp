@@ -33,12 +33,26 @@
 
 namespace mkpp {
 
+// The second-derivative pass keeps nu_g . rbar_s for every stage and group,
+// (S + 1) x (NG + 1) doubles.  It reuses the LU block, which is free by then,
+// when that block is large enough (saprc99: 2015 >= 1055); a small mechanism's
+// LU block is not, and the pass gets an area of its own behind the layout.
+template <class W, class Tab>
+constexpr int adjoint_wh_doubles() {
+  return (Tab::S + 1) * (W::NG + 1);
+}
+template <class W, class Tab>
+constexpr bool adjoint_wh_in_lu() {
+  return adjoint_wh_doubles<W, Tab>() <= W::SPAN + 1;
+}
+
 // LDS doubles of the adjoint: the forward layout plus rbar and K of every
-// stage, the u vector of a J^T pass and the fixed-species adjoint.
+// stage, the u vector of a J^T pass and the fixed-species adjoint (and the
+// second-derivative area where the LU block cannot hold it).
 template <class W, class Tab>
 constexpr int adjoint_lds_doubles() {
   return W::LDS_DOUBLES + (2 * Tab::S + 1) * (W::NV + 1) + Tab::S * W::NY + (W::NV + 1) + (W::NF + 1) + W::NY +
-         (W::NG + 1) + WaveTables<W>::Code::SEXTRA;
+         (W::NG + 1) + WaveTables<W>::Code::SEXTRA + (adjoint_wh_in_lu<W, Tab>() ? 0 : adjoint_wh_doubles<W, Tab>());
 }
 
 template <class W, class Tab>
@@ -58,7 +72,8 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
   double* fbar = uv + (NV + 1);      // [NF]      adjoint of the fixed species
   double* sacc = fbar + (NF + 1);    // [NY]      with wj (+ SEXTRA): scatter chunk partials
   double* wj = sacc + NY;            // [NG+1]    nu_g . u of a J^T pass
-  double* wh = lu;                   // [S+1][NG+1] nu_g . rbv rows (second-derivative pass, after the LU is done)
+  // [S+1][NG+1] nu_g . rbv rows (second-derivative pass, after the LU is done)
+  double* wh = adjoint_wh_in_lu<W, Tab>() ? lu : wj + (NG + 1) + C_::SEXTRA;
 
   // lane slot (b, lane) holds row ROWOF[b * 64 + lane] (255: none)
   int rb[NB], rowv[NB];
@@ -307,7 +322,10 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
       if (gi < NG) {
         double pw[3], pm1[3], pm2[3];
         monomial(gi, pw, pm1, pm2);
-        const uint8_t* sl = &T_::GPOWS[gi * 3];
+        // K by the entry's own row: a unit-value species reads 1.0 in the
+        // monomial (as W is assembled) but its Jacobian column multiplies its
+        // stage increment
+        const uint8_t* sk = &T_::GPOWK[gi * 3];
         // first and second derivatives of the monomial (without k)
         double d1[3], d2[3][3];
 #pragma unroll
@@ -345,7 +363,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
           const double w = ws[s];
           double kvs[3];
 #pragma unroll
-          for (int e = 0; e < 3; ++e) kvs[e] = kv[s * NY + sl[e]];
+          for (int e = 0; e < 3; ++e) kvs[e] = kv[s * NY + sk[e]];
           double dk = 0.0;
 #pragma unroll
           for (int e = 0; e < 3; ++e) {

@@ -22,6 +22,7 @@ __device__ const uint8_t GFAC[3] = {0, 1, 1};
 __device__ const uint8_t GNPOW[1] = {1};
 __device__ const uint8_t GPOWS[3] = {0, 1, 1};
 __device__ const uint8_t GPOWP[3] = {1, 0, 0};
+__device__ const uint8_t GPOWK[3] = {0, 1, 1};
 __device__ const uint16_t POFF[2] = {0, 1};
 __device__ const double COEF[2] = {-1.0, 0.0};
 __device__ const uint16_t FOFS[2] = {0, 1};

@@ -91,8 +91,9 @@ int launch_eval_jacobian(const EvalArgs& a, hipStream_t s) {
 
 // Defines mkpp_ops_<name>() for a lane-per-cell mechanism whose generated
 // headers declare mkpp::gen::<name>::Mech and mkpp::gen::<name>_unrolled::Unrolled
-// (the unit includes unrolled.hpp).
-#define MKPP_LANE_MECHANISM(NAME, F0, F1, F2, F3, F4)                                                        \
+// (the unit includes unrolled.hpp).  DIFF: the differentiation entry (the wave
+// engine's launch_wave_diff over the same mechanism's wave tables).
+#define MKPP_LANE_MECHANISM(NAME, F0, F1, F2, F3, F4, DIFF)                                                  \
   namespace mkpp {                                                                                         \
   const MechOps* mkpp_ops_##NAME() {                                                                       \
     using M = gen::NAME::Mech;                                                                             \
@@ -108,7 +109,7 @@ int launch_eval_jacobian(const EvalArgs& a, hipStream_t s) {
                                 &launch_eval_rates<M>,                                                     \
                                 &launch_eval_jacobian<M>,                                                  \
                                 {F0, F1, F2, F3, F4},                                                      \
-                                nullptr,                                                                   \
+                                DIFF,                                                                      \
                                 M::NG,                                                                     \
                                 M::NV,                                                                     \
                                 &launch_lane_reduced<M>,                                                   \

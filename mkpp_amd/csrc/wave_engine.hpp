@@ -254,7 +254,10 @@ __device__ __forceinline__ void wave_tlm_group_jvp(const double* __restrict__ y,
 
 // scr[g] = k_g sum_e sum_f d2_ef(y) K_e dy_f + dk_g sum_e d1_e(y) K_e: with
 // wave_row_sums, the derivative of J(y) K along (dy, dk).  y, dy, K by y slot
-// (K of fixed species and ONE is 0).
+// (K of fixed species and ONE is 0).  The monomial is evaluated as W is
+// assembled, a unit-value species reading 1.0 (slot ONE); its Jacobian column
+// still multiplies its own stage increment, so K is read through GPOWK (the
+// species' row) rather than GPOWS.
 template <class W, int NGP>
 __device__ __forceinline__ void wave_tlm_group_djk(const double* __restrict__ y, const double* __restrict__ dy,
                                                    const double* __restrict__ kv, double* __restrict__ scr,
@@ -268,10 +271,11 @@ __device__ __forceinline__ void wave_tlm_group_djk(const double* __restrict__ y,
       double mono, d1[3], d2[3][3];
       wave_monomial_derivs<W>(y, g, mono, d1, d2);
       const uint8_t* sl = &T_::GPOWS[g * 3];
+      const uint8_t* sk = &T_::GPOWK[g * 3];
       double kk[3], dd[3];
 #pragma unroll
       for (int e = 0; e < 3; ++e) {
-        kk[e] = kv[sl[e]];
+        kk[e] = kv[sk[e]];
         dd[e] = dy[sl[e]];
       }
       double h = 0.0, jk = 0.0;
@@ -984,6 +988,7 @@ int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
     static constexpr const uint8_t* GFAC = gen::NAME##_wave::GFAC;                                          \
     static constexpr const uint8_t* GNPOW = gen::NAME##_wave::GNPOW;                                        \
     static constexpr const uint8_t* GPOWS = gen::NAME##_wave::GPOWS;                                        \
+    static constexpr const uint8_t* GPOWK = gen::NAME##_wave::GPOWK;                                        \
     static constexpr const uint8_t* GPOWP = gen::NAME##_wave::GPOWP;                                        \
     static constexpr const uint16_t* POFF = gen::NAME##_wave::POFF;                                         \
     static constexpr const double* COEF = gen::NAME##_wave::COEF;                                           \

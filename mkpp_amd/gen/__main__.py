@@ -47,6 +47,8 @@ def main(argv=None):
             nv = len(p["species"]) - sum(p["fixed"])
             if nv <= emit_hip.LANE_MAX_VARIABLE_SPECIES and name not in FORCE_WAVE:
                 emit_hip.write(mech, p, outdir)
+                # the wave engine's tables serve the lane mechanism's differentiation
+                emit_wave.write_headers(mech, p, outdir)
                 engines[name] = (name, "lane", 0)
             else:
                 emit_wave.write(mech, p, outdir)

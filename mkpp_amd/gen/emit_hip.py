@@ -417,12 +417,16 @@ def emit_unit(p: dict) -> str:
     sf = p["step_flops"]
     flops = ", ".join(f"{float(sf[m])!r}" for m in ("ros2", "ros3", "ros4", "rodas3", "rodas4"))
     return (
-        f"// Generated by mkpp_amd.gen.emit_hip: registers {name} (one cell per lane).\n"
+        f"// Generated by mkpp_amd.gen.emit_hip: registers {name} (one cell per lane;\n"
+        "// differentiation on the wave engine's tables of the same mechanism).\n"
         f'#include "{name}.hpp"\n'
         '#include "../runtime.hpp"\n'
         '#include "../unrolled.hpp"\n'
-        f'#include "{name}_unrolled.hpp"\n\n'
-        f"MKPP_LANE_MECHANISM({name}, {flops})\n"
+        '#include "../adjoint.hpp"\n'
+        f'#include "{name}_unrolled.hpp"\n'
+        f'#include "{name}_wcode.hpp"\n\n'
+        f"MKPP_WAVE_TABLES({name})\n"
+        f"MKPP_LANE_MECHANISM({name}, {flops}, (&launch_wave_diff<gen::{name}_wave::Wave>))\n"
     )
 
 

@@ -68,6 +68,24 @@ def row_displacement(nv: int, pattern: list[list[int]], bank: list[int] | None =
     return R, span, lane
 
 
+def _numeric_law_params(g: dict) -> list | None:
+    """The law's parameters as numbers, or None.  One expression form has an
+    exact numeric equivalent: an Arrhenius pre-exponential A = a exp(-b / T)
+    with B = C = 0 is the law's own A = a, C = -b (the device law computes
+    a * exp(-b / T), the same operations as the expression)."""
+    params = g["params"]
+    if all(isinstance(v, float) for v in params.values()):
+        return [float(v) for v in params.values()]
+    if g["law"] == "ARRHENIUS" and params["B"] == 0.0 and params["C"] == 0.0:
+        r = g["rpn"]["A"]
+        if len(r) == 7 and r[2:] == ["neg", "T", "/", "exp", "*"]:
+            try:
+                return [float(r[0]), 0.0, -float(r[1])]
+            except ValueError:
+                return None
+    return None
+
+
 # LDS doubles per wavefront that keep 8 wavefronts (2 per SIMD) on a CU's 160 KB
 LDS_DOUBLES_8_WAVES = 160 * 1024 // 8 // 8
 
@@ -162,7 +180,7 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
     # flux factors: squared species first so the product equals the oracle's
     # y_a * (y_b * y_b) (IEEE multiplication commutes)
     gfac = []  # [ng][3] y slots
-    gpow_s, gpow_p, gnpow = [], [], []  # reactant entries in plan order
+    gpow_s, gpow_p, gnpow, gpow_k = [], [], [], []  # reactant entries in plan order
     law, photo, params = [], [], []
     for g in groups:
         ents = [(s, q) for s, q in g["powers"]]
@@ -175,13 +193,17 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
         gfac.append(facs)
         ss = [yslot(s) for s, _ in ents] + [ONE] * (MAX_FACTORS - len(ents))
         pp = [q for _, q in ents] + [0] * (MAX_FACTORS - len(ents))
+        # where the entry's stage increment K lives: a unit-value species reads
+        # as 1.0 in the flux (slot ONE) but is a variable row with its own K
+        kk = [vm.v_of_phys[s] if s in vm.v_of_phys else yslot(s) for s, _ in ents] + [ONE] * (MAX_FACTORS - len(ents))
+        gpow_k.append(kk)
         gpow_s.append(ss)
         gpow_p.append(pp)
         gnpow.append(len(ents))
         law.append(LAW_IDS[g["law"]])
         photo.append(g["photo"])
-        vals = [float(v) if isinstance(v, float) else None for v in g["params"].values()]
-        if any(v is None for v in vals):
+        vals = _numeric_law_params(g)
+        if vals is None:
             raise ValueError("expression-valued rate parameters are handled by the lane engine only")
         params.append((vals + [0.0] * 7)[:7])
     # partial slots: one per (group, reactant entry)
@@ -317,7 +339,7 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
         "var_phys": vm.var_phys, "fixed_phys": vm.fixed_phys, "third_body": p["third_body"],
         "slot": slot, "rowof": rowof, "k0": k0, "R": R, "lcnt": lcnt, "lofs": lofs, "lrow": lrow, "ucnt": ucnt, "uofs": uofs, "ucol": ucol,
         "fmask": fmask, "bmask": bmask, "law": law, "photo": photo, "params": params, "gfac": gfac,
-        "gnpow": gnpow, "gpow_s": gpow_s, "gpow_p": gpow_p, "poff": poff, "coefs": coefs,
+        "gnpow": gnpow, "gpow_s": gpow_s, "gpow_p": gpow_p, "gpow_k": gpow_k, "poff": poff, "coefs": coefs,
         "fofs": fofs, "fg": fg, "fc": fc, "waddr": waddr, "wdiag": wdiag, "wofs": wofs, "wpi": wpi, "wci": wci,
         "nmwp": nmwp, "mw_addr": mw_addr, "mw_row": mw_row, "mw_col": mw_col, "tolidx": list(vm.var_log),
         "nentries": len(entries), "num_reactions": p["num_reactions"],
@@ -368,6 +390,7 @@ def emit(t: dict) -> str:
         _arr("uint8_t", "GNPOW", t["gnpow"]),
         _arr("uint8_t", "GPOWS", [v for row in t["gpow_s"] for v in row]),
         _arr("uint8_t", "GPOWP", [v for row in t["gpow_p"] for v in row]),
+        _arr("uint8_t", "GPOWK", [v for row in t["gpow_k"] for v in row]),
         _arr("uint16_t", "POFF", t["poff"]),
         _arr("double", "COEF", t["coefs"]),
         _arr("uint16_t", "FOFS", t["fofs"]),
@@ -427,19 +450,28 @@ def emit_unit(t: dict) -> str:
     )
 
 
-def write(mech: Mechanism, p: dict, outdir: str) -> list:
+def write_headers(mech: Mechanism, p: dict, outdir: str) -> list:
+    """<mech>_wave.hpp and <mech>_wcode.hpp only: the wave engine's tables and
+    code, for a unit that registers the mechanism itself (the lane-per-cell
+    mechanisms use them for differentiation)."""
     t = build_tables(mech, p)
     ct = build_code_tables(t)  # may append 0.0 to the coefficient table
     os.makedirs(outdir, exist_ok=True)
     a = os.path.join(outdir, f"{t['name']}_wave.hpp")
     with open(a, "w") as f:
         f.write(emit(t))
-    b = os.path.join(outdir, f"mech_{t['name']}.hip")
-    with open(b, "w") as f:
-        f.write(emit_unit(t))
     c = os.path.join(outdir, f"{t['name']}_wcode.hpp")
     with open(c, "w") as f:
         f.write(emit_code(t, ct))
+    return [a, c]
+
+
+def write(mech: Mechanism, p: dict, outdir: str) -> list:
+    t = build_tables(mech, p)
+    a, c = write_headers(mech, p, outdir)
+    b = os.path.join(outdir, f"mech_{t['name']}.hip")
+    with open(b, "w") as f:
+        f.write(emit_unit(t))
     return [a, b, c]
 
 

@@ -84,12 +84,20 @@ class DenseMechanism:
             np.add.at(D, (slice(None), np.arange(self.ng), self.s[:, e]), k * d1[..., e])
         return np.einsum("ig,cgj->cij", self.nu, D[..., :self.n])
 
+    def _dy_flux(self, dy):
+        """dy as the fluxes see it: a unit-value species reads as the constant
+        1.0, so its direction does not enter any monomial."""
+        d = np.array(dy, dtype=np.float64, copy=True)
+        d[:, self.unit] = 0.0
+        return d
+
     def djac_dir(self, y, k, K, dy, dk):
-        """(d/dy J . dy + d/dk J . dk) K for every cell: (cells, N)."""
+        """(d/dy J . dy + d/dk J . dk) K for every cell: (cells, N).  J's
+        columns of unit-value species (evaluated at 1.0) multiply their own K."""
         pw, d1, d2 = self.monomial(y)
         z = np.zeros((y.shape[0], 1))
         Ke = np.concatenate([K, z], axis=1)[:, self.s]      # (cells, NG, 3)
-        de = np.concatenate([dy, z], axis=1)[:, self.s]
+        de = np.concatenate([self._dy_flux(dy), z], axis=1)[:, self.s]
         second = np.einsum("cgef,cge,cgf->cg", d2, Ke, de)
         first = np.einsum("cge,cge->cg", d1, Ke)
         return (k * second + dk * first) @ self.nu.T
@@ -98,7 +106,7 @@ class DenseMechanism:
         """dF = J dy + dF/dk dk."""
         pw, d1, _ = self.monomial(y)
         z = np.zeros((y.shape[0], 1))
-        de = np.concatenate([dy, z], axis=1)[:, self.s]
+        de = np.concatenate([self._dy_flux(dy), z], axis=1)[:, self.s]
         return (k * np.einsum("cge,cge->cg", d1, de) + dk * pw.prod(axis=2)) @ self.nu.T
 
 
