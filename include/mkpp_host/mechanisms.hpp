@@ -39,6 +39,10 @@
 #include "decay_wave.hpp"
 #define MKPP_HOST_HAS_DECAY_WAVE 1
 #endif
+#if __has_include("ts1.hpp")
+#include "ts1.hpp"
+#define MKPP_HOST_HAS_TS1 1
+#endif
 
 namespace mkpp::host::detail {
 
@@ -107,6 +111,16 @@ inline bool generated_options(const std::string& name, SolverOptions& o) {
 #ifdef MKPP_HOST_HAS_DECAY_WAVE
   if (name == "decay_wave") {
     using K = ::mkpp::generated::decay_wave::SolverKernels<>;
+    o.method = K::METHOD;
+    o.atol = K::atol;
+    o.rtol = K::rtol;
+    o.initial_step_seed = K{}.initial_step_seed;
+    return true;
+  }
+#endif
+#ifdef MKPP_HOST_HAS_TS1
+  if (name == "ts1") {
+    using K = ::mkpp::generated::ts1::SolverKernels<>;
     o.method = K::METHOD;
     o.atol = K::atol;
     o.rtol = K::rtol;

@@ -223,7 +223,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
         for (int s = 0; s < NVEC; ++s) acc[s] = fma(cf, u[s * (NV + 1) + r], acc[s]);
       }
       const int gg = T_::CG_GRP[p * 64 + lane];
-      if (gg != 255) {
+      if (gg != 0xFFFF) {
 #pragma unroll
         for (int s = 0; s < NVEC; ++s) out[s * (NG + 1) + gg] += acc[s];
       }
@@ -607,10 +607,21 @@ int launch_wave_diff(int mode, const DeviceArgs& a, const AdjArgs& g, hipStream_
   }
   const size_t fwd_lds = sizeof(double) * W::LDS_DOUBLES;
 #define MKPP_DIFF_CASE(TAB)                                                                              \
-  if (mode == 1) wave_record_kernel<W, TAB, 1><<<grid, 64, fwd_lds, s>>>(a, g, base);                    \
-  else if (mode == 2) wave_record_kernel<W, TAB, 2><<<grid, 64, fwd_lds, s>>>(a, g, base);               \
-  else if (mode == 4) wave_tlm_kernel<W, TAB><<<grid, 64, sizeof(double) * wave_tlm_lds_doubles<W>(), s>>>(a, g, base); \
-  else wave_adjoint_kernel<W, TAB><<<grid, 64, sizeof(double) * adjoint_lds_doubles<W, TAB>(), s>>>(a, g, base);
+  if (mode == 1) {                                                                                       \
+    wave_allow_lds(&wave_record_kernel<W, TAB, 1>, fwd_lds);                                             \
+    wave_record_kernel<W, TAB, 1><<<grid, 64, fwd_lds, s>>>(a, g, base);                                 \
+  } else if (mode == 2) {                                                                                \
+    wave_allow_lds(&wave_record_kernel<W, TAB, 2>, fwd_lds);                                             \
+    wave_record_kernel<W, TAB, 2><<<grid, 64, fwd_lds, s>>>(a, g, base);                                 \
+  } else if (mode == 4) {                                                                                \
+    const size_t tl = sizeof(double) * wave_tlm_lds_doubles<W>();                                        \
+    wave_allow_lds(&wave_tlm_kernel<W, TAB>, tl);                                                        \
+    wave_tlm_kernel<W, TAB><<<grid, 64, tl, s>>>(a, g, base);                                            \
+  } else {                                                                                               \
+    const size_t al = sizeof(double) * adjoint_lds_doubles<W, TAB>();                                    \
+    wave_allow_lds(&wave_adjoint_kernel<W, TAB>, al);                                                    \
+    wave_adjoint_kernel<W, TAB><<<grid, 64, al, s>>>(a, g, base);                                        \
+  }
   if (a.method < ROS2 || a.method > RODAS4) return -1;
   for (long base = 0; base < a.ncells; base += WAVE_CHUNK_CELLS) {
     const unsigned grid = wave_chunk_grid(a.ncells, base);

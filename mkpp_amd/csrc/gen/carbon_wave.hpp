@@ -26,7 +26,7 @@ __device__ const uint8_t GPOWK[15] = {8, 6, 11, 8, 10, 11, 3, 11, 11, 4, 6, 11, 
 __device__ const uint16_t POFF[6] = {0, 2, 4, 5, 7, 8};
 __device__ const double COEF[3] = {-1.0, 1.0, 0.0};
 __device__ const uint16_t FOFS[12] = {0, 1, 2, 3, 4, 7, 8, 10, 11, 13, 14, 15};
-__device__ const uint8_t FG[15] = {4, 2, 4, 2, 2, 3, 4, 3, 0, 3, 0, 0, 1, 1, 1};
+__device__ const uint16_t FG[15] = {4, 2, 4, 2, 2, 3, 4, 3, 0, 3, 0, 0, 1, 1, 1};
 __device__ const uint8_t FC[15] = {1, 1, 0, 0, 1, 0, 1, 1, 0, 0, 1, 0, 0, 1, 0};
 __device__ const uint16_t WADDR[27] = {12, 20, 13, 21, 5, 27, 2, 3, 4, 6, 7, 9, 10, 14, 15, 17, 18, 22, 23, 25, 26, 28, 11, 19, 8, 16, 24};
 __device__ const uint8_t WDIAG[27] = {1, 1, 0, 0, 1, 1, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1};
@@ -57,6 +57,10 @@ struct Wave {
   static constexpr int N = 11, NV = 11, NF = 0, NG = 5;
   static constexpr int NPH = 0, NB = 1, SPAN = 29, NY = 12;
   static constexpr int NPART = 8, ONE = 11, NENTRIES = 27;
+  static constexpr int NFAC = 3;  // factor slots per flux group (GFAC)
+  // wavefronts per SIMD the kernels are built for: 2 where the LDS block lets
+  // 8 share a CU, else 1 (then spills go to AGPRs, not scratch)
+  static constexpr int WAVES_PER_EU = 2;
   static constexpr int NMWP = 1;  // reduction variant: passes over the W locations
   static constexpr int THIRD_BODY = -1, NUM_REACTIONS = 5;
   static constexpr bool SUPPORTS_RUNTIME_ENVIRONMENT = true;

@@ -26,7 +26,7 @@ __device__ const uint8_t GPOWK[12] = {2, 4, 4, 1, 2, 3, 0, 4, 4, 1, 0, 4};
 __device__ const uint16_t POFF[5] = {0, 1, 4, 5, 7};
 __device__ const double COEF[4] = {2.0, -1.0, 1.0, 0.0};
 __device__ const uint16_t FOFS[4] = {0, 3, 7, 11};
-__device__ const uint8_t FG[11] = {1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3};
+__device__ const uint16_t FG[11] = {1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3};
 __device__ const uint8_t FC[11] = {2, 1, 1, 0, 1, 2, 1, 1, 1, 2, 0};
 __device__ const uint16_t WADDR[9] = {0, 1, 3, 4, 5, 6, 7, 8, 2};
 __device__ const uint8_t WDIAG[9] = {1, 0, 0, 1, 0, 0, 0, 1, 0};
@@ -57,6 +57,10 @@ struct Wave {
   static constexpr int N = 4, NV = 3, NF = 1, NG = 4;
   static constexpr int NPH = 2, NB = 1, SPAN = 9, NY = 5;
   static constexpr int NPART = 7, ONE = 4, NENTRIES = 9;
+  static constexpr int NFAC = 3;  // factor slots per flux group (GFAC)
+  // wavefronts per SIMD the kernels are built for: 2 where the LDS block lets
+  // 8 share a CU, else 1 (then spills go to AGPRs, not scratch)
+  static constexpr int WAVES_PER_EU = 2;
   static constexpr int NMWP = 1;  // reduction variant: passes over the W locations
   static constexpr int THIRD_BODY = 3, NUM_REACTIONS = 4;
   static constexpr bool SUPPORTS_RUNTIME_ENVIRONMENT = false;

@@ -26,7 +26,7 @@ __device__ const uint8_t GPOWK[3] = {0, 1, 1};
 __device__ const uint16_t POFF[2] = {0, 1};
 __device__ const double COEF[2] = {-1.0, 0.0};
 __device__ const uint16_t FOFS[2] = {0, 1};
-__device__ const uint8_t FG[1] = {0};
+__device__ const uint16_t FG[1] = {0};
 __device__ const uint8_t FC[1] = {0};
 __device__ const uint16_t WADDR[1] = {0};
 __device__ const uint8_t WDIAG[1] = {1};
@@ -57,6 +57,10 @@ struct Wave {
   static constexpr int N = 1, NV = 1, NF = 0, NG = 1;
   static constexpr int NPH = 1, NB = 1, SPAN = 1, NY = 2;
   static constexpr int NPART = 1, ONE = 1, NENTRIES = 1;
+  static constexpr int NFAC = 3;  // factor slots per flux group (GFAC)
+  // wavefronts per SIMD the kernels are built for: 2 where the LDS block lets
+  // 8 share a CU, else 1 (then spills go to AGPRs, not scratch)
+  static constexpr int WAVES_PER_EU = 2;
   static constexpr int NMWP = 1;  // reduction variant: passes over the W locations
   static constexpr int THIRD_BODY = -1, NUM_REACTIONS = 1;
   static constexpr bool SUPPORTS_RUNTIME_ENVIRONMENT = false;

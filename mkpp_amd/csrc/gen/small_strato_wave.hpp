@@ -26,7 +26,7 @@ __device__ const uint8_t GPOWK[30] = {2, 7, 7, 1, 2, 7, 3, 7, 7, 1, 3, 7, 3, 7, 
 __device__ const uint16_t POFF[11] = {0, 1, 3, 4, 6, 7, 9, 11, 13, 15, 16};
 __device__ const double COEF[4] = {2.0, -1.0, 1.0, 0.0};
 __device__ const uint16_t FOFS[7] = {0, 3, 10, 18, 24, 27, 30};
-__device__ const uint8_t FG[30] = {4, 5, 6, 0, 1, 2, 3, 5, 8, 9, 0, 1, 2, 3, 4, 6, 7, 8, 1, 2, 3, 4, 6, 7, 7, 8, 9, 7, 8, 9};
+__device__ const uint16_t FG[30] = {4, 5, 6, 0, 1, 2, 3, 5, 8, 9, 0, 1, 2, 3, 4, 6, 7, 8, 1, 2, 3, 4, 6, 7, 7, 8, 9, 7, 8, 9};
 __device__ const uint8_t FC[30] = {2, 1, 1, 0, 1, 2, 1, 2, 1, 2, 1, 1, 2, 0, 2, 0, 2, 2, 2, 1, 1, 1, 1, 1, 1, 2, 2, 2, 1, 1};
 __device__ const uint16_t WADDR[29] = {3, 9, 13, 1, 28, 31, 14, 15, 17, 2, 7, 22, 27, 12, 0, 4, 5, 6, 8, 10, 18, 20, 21, 23, 25, 26, 11, 19, 24};
 __device__ const uint8_t WDIAG[29] = {0, 1, 1, 0, 1, 0, 0, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0};
@@ -57,6 +57,10 @@ struct Wave {
   static constexpr int N = 7, NV = 6, NF = 1, NG = 10;
   static constexpr int NPH = 4, NB = 1, SPAN = 32, NY = 8;
   static constexpr int NPART = 16, ONE = 7, NENTRIES = 29;
+  static constexpr int NFAC = 3;  // factor slots per flux group (GFAC)
+  // wavefronts per SIMD the kernels are built for: 2 where the LDS block lets
+  // 8 share a CU, else 1 (then spills go to AGPRs, not scratch)
+  static constexpr int WAVES_PER_EU = 2;
   static constexpr int NMWP = 1;  // reduction variant: passes over the W locations
   static constexpr int THIRD_BODY = 5, NUM_REACTIONS = 10;
   static constexpr bool SUPPORTS_RUNTIME_ENVIRONMENT = false;

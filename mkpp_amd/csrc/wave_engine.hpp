@@ -147,9 +147,11 @@ __device__ __forceinline__ void wave_row_sums(double* __restrict__ scr, const bo
     double acc = 0.0;
 #pragma unroll
     for (int q = 0; q < C_::TF / 2; ++q) {
+      // two 16-bit terms, flux slot in the low FBITS bits, coefficient above
+      constexpr uint32_t FB = C_::FBITS, FM = (1u << FB) - 1u;
       const uint32_t two = tw[q];
-      acc = fma(T_::COEF[(two >> 8) & 0xFFu], scr[two & 0xFFu], acc);
-      acc = fma(T_::COEF[two >> 24], scr[(two >> 16) & 0xFFu], acc);
+      acc = fma(T_::COEF[(two & 0xFFFFu) >> FB], scr[two & FM], acc);
+      acc = fma(T_::COEF[two >> (16 + FB)], scr[(two >> 16) & FM], acc);
     }
     scr[C_::PBASE + p * 64 + lane] = acc;
   }
@@ -173,8 +175,12 @@ __device__ __forceinline__ void wave_eval_f(const double* __restrict__ ylds, dou
   for (int p = 0; p < NGP; ++p) {
     const int g = p * 64 + lane;
     if (g < NG) {
-      const uint8_t* fa = &T_::GFAC[g * 3];
-      scr[g] = kreg[p] * (((1.0 * ylds[fa[0]]) * ylds[fa[1]]) * ylds[fa[2]]);
+      // ((1.0 * y_0) * y_1) * ... over the NFAC factor slots (padding reads 1.0)
+      const uint8_t* fa = &T_::GFAC[g * W::NFAC];
+      double m = 1.0 * ylds[fa[0]];
+#pragma unroll
+      for (int e = 1; e < W::NFAC; ++e) m = m * ylds[fa[e]];
+      scr[g] = kreg[p] * m;
     }
   }
   if (lane == 0) scr[NG] = 0.0;
@@ -182,7 +188,8 @@ __device__ __forceinline__ void wave_eval_f(const double* __restrict__ ylds, dou
 }
 
 // Tangent-linear group values (MODE 4 of wave_integrate_cell).  Fluxes are
-// mass-action monomials r_g = k_g prod_e y_e^q_e over at most three entries.
+// mass-action monomials r_g = k_g prod_e y_e^q_e over at most three entries
+// (distinct reactants; powers up to 3).
 // Derivatives of the monomial at the state y (by y slot): d1[e] = d/dy_e,
 // d2[e][f] = d2/dy_e dy_f (the power rule on each entry).
 template <class W>
@@ -811,14 +818,16 @@ __device__ inline long wave_cell_of_block(long base) {
 // library's polynomial constants out of, so they are not kept live (and
 // spilled to scratch, 8.7 KB per wave) across the whole integration.
 template <class W, class Tab>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) wave_integrate_kernel(DeviceArgs a, long base) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W::WAVES_PER_EU, W::WAVES_PER_EU)))
+wave_integrate_kernel(DeviceArgs a, long base) {
   extern __shared__ double lds[];
   const long cell = wave_cell_of_block(base);
   if (cell < a.ncells) wave_integrate_cell<W, Tab>(a, cell, lds, threadIdx.x);
 }
 
 template <class W, class Tab>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) wave_reduced_kernel(DeviceArgs a, long base) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W::WAVES_PER_EU, W::WAVES_PER_EU)))
+wave_reduced_kernel(DeviceArgs a, long base) {
   extern __shared__ double lds[];
   const long cell = wave_cell_of_block(base);
   if (cell < a.ncells) wave_integrate_cell<W, Tab, 3>(a, cell, lds, threadIdx.x);
@@ -921,6 +930,15 @@ __global__ void __launch_bounds__(64) wave_eval_kernel(EvalArgs a, int what) {
   }
 }
 
+// A launch above 64 KB of dynamic LDS (ts1: 66 KB per wavefront) declares its
+// size to the runtime first; gfx950 gives a workgroup up to 160 KB.
+template <class K>
+inline void wave_allow_lds(K* kernel, size_t bytes) {
+  if (bytes > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)bytes);
+}
+
 template <class W>
 int launch_wave_integrate(const DeviceArgs& a, hipStream_t s) {
   if (a.ncells <= 0) return 0;
@@ -929,11 +947,26 @@ int launch_wave_integrate(const DeviceArgs& a, hipStream_t s) {
   for (long base = 0; base < a.ncells; base += WAVE_CHUNK_CELLS) {
     const unsigned grid = wave_chunk_grid(a.ncells, base);
     switch (a.method) {
-      case ROS2: wave_integrate_kernel<W, TabRos2><<<grid, 64, lds, s>>>(a, base); break;
-      case ROS3: wave_integrate_kernel<W, TabRos3><<<grid, 64, lds, s>>>(a, base); break;
-      case ROS4: wave_integrate_kernel<W, TabRos4><<<grid, 64, lds, s>>>(a, base); break;
-      case RODAS3: wave_integrate_kernel<W, TabRodas3><<<grid, 64, lds, s>>>(a, base); break;
-      case RODAS4: wave_integrate_kernel<W, TabRodas4><<<grid, 64, lds, s>>>(a, base); break;
+      case ROS2:
+        wave_allow_lds(&wave_integrate_kernel<W, TabRos2>, lds);
+        wave_integrate_kernel<W, TabRos2><<<grid, 64, lds, s>>>(a, base);
+        break;
+      case ROS3:
+        wave_allow_lds(&wave_integrate_kernel<W, TabRos3>, lds);
+        wave_integrate_kernel<W, TabRos3><<<grid, 64, lds, s>>>(a, base);
+        break;
+      case ROS4:
+        wave_allow_lds(&wave_integrate_kernel<W, TabRos4>, lds);
+        wave_integrate_kernel<W, TabRos4><<<grid, 64, lds, s>>>(a, base);
+        break;
+      case RODAS3:
+        wave_allow_lds(&wave_integrate_kernel<W, TabRodas3>, lds);
+        wave_integrate_kernel<W, TabRodas3><<<grid, 64, lds, s>>>(a, base);
+        break;
+      case RODAS4:
+        wave_allow_lds(&wave_integrate_kernel<W, TabRodas4>, lds);
+        wave_integrate_kernel<W, TabRodas4><<<grid, 64, lds, s>>>(a, base);
+        break;
     }
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
@@ -947,11 +980,26 @@ int launch_wave_reduced(const DeviceArgs& a, hipStream_t s) {
   for (long base = 0; base < a.ncells; base += WAVE_CHUNK_CELLS) {
     const unsigned grid = wave_chunk_grid(a.ncells, base);
     switch (a.method) {
-      case ROS2: wave_reduced_kernel<W, TabRos2><<<grid, 64, lds, s>>>(a, base); break;
-      case ROS3: wave_reduced_kernel<W, TabRos3><<<grid, 64, lds, s>>>(a, base); break;
-      case ROS4: wave_reduced_kernel<W, TabRos4><<<grid, 64, lds, s>>>(a, base); break;
-      case RODAS3: wave_reduced_kernel<W, TabRodas3><<<grid, 64, lds, s>>>(a, base); break;
-      case RODAS4: wave_reduced_kernel<W, TabRodas4><<<grid, 64, lds, s>>>(a, base); break;
+      case ROS2:
+        wave_allow_lds(&wave_reduced_kernel<W, TabRos2>, lds);
+        wave_reduced_kernel<W, TabRos2><<<grid, 64, lds, s>>>(a, base);
+        break;
+      case ROS3:
+        wave_allow_lds(&wave_reduced_kernel<W, TabRos3>, lds);
+        wave_reduced_kernel<W, TabRos3><<<grid, 64, lds, s>>>(a, base);
+        break;
+      case ROS4:
+        wave_allow_lds(&wave_reduced_kernel<W, TabRos4>, lds);
+        wave_reduced_kernel<W, TabRos4><<<grid, 64, lds, s>>>(a, base);
+        break;
+      case RODAS3:
+        wave_allow_lds(&wave_reduced_kernel<W, TabRodas3>, lds);
+        wave_reduced_kernel<W, TabRodas3><<<grid, 64, lds, s>>>(a, base);
+        break;
+      case RODAS4:
+        wave_allow_lds(&wave_reduced_kernel<W, TabRodas4>, lds);
+        wave_reduced_kernel<W, TabRodas4><<<grid, 64, lds, s>>>(a, base);
+        break;
     }
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
@@ -993,7 +1041,7 @@ int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
     static constexpr const uint16_t* POFF = gen::NAME##_wave::POFF;                                         \
     static constexpr const double* COEF = gen::NAME##_wave::COEF;                                           \
     static constexpr const uint16_t* FOFS = gen::NAME##_wave::FOFS;                                         \
-    static constexpr const uint8_t* FG = gen::NAME##_wave::FG;                                              \
+    static constexpr const uint16_t* FG = gen::NAME##_wave::FG;                                             \
     static constexpr const uint8_t* FC = gen::NAME##_wave::FC;                                              \
     static constexpr const uint16_t* WADDR = gen::NAME##_wave::WADDR;                                       \
     static constexpr const uint8_t* WDIAG = gen::NAME##_wave::WDIAG;                                        \
@@ -1023,13 +1071,35 @@ int launch_wave_eval(const EvalArgs& a, hipStream_t s, int what) {
     static constexpr const uint16_t* CS_PART = gen::NAME##_wave::CS_PART;                                   \
     static constexpr const uint16_t* CS_FIX = gen::NAME##_wave::CS_FIX;                                     \
     static constexpr const uint16_t* CG_TERM = gen::NAME##_wave::CG_TERM;                                   \
-    static constexpr const uint8_t* CG_GRP = gen::NAME##_wave::CG_GRP;                                      \
+    static constexpr const uint16_t* CG_GRP = gen::NAME##_wave::CG_GRP;                                     \
     static constexpr const uint16_t* MW_ADDR = gen::NAME##_wave::MW_ADDR;                                   \
     static constexpr const uint8_t* MW_RSLOT = gen::NAME##_wave::MW_RSLOT;                                  \
     static constexpr const uint8_t* MW_CSLOT = gen::NAME##_wave::MW_CSLOT;                                  \
     static constexpr const uint8_t* TOLIDX = gen::NAME##_wave::TOLIDX;                                      \
     using Code = gen::NAME##_wave::Code;                                                                    \
   };                                                                                                        \
+  }
+
+// Registers a generated wave mechanism without the differentiation kernels
+// (the diff entry points return -103 for it).
+#define MKPP_WAVE_MECHANISM_NODIFF(NAME, F0, F1, F2, F3, F4)                                                  \
+  MKPP_WAVE_TABLES(NAME)                                                                                    \
+  namespace mkpp {                                                                                          \
+  static int wave_eval_rates_##NAME(const EvalArgs& a, hipStream_t s) {                                     \
+    return launch_wave_eval<gen::NAME##_wave::Wave>(a, s, 0);                                               \
+  }                                                                                                         \
+  static int wave_eval_jac_##NAME(const EvalArgs& a, hipStream_t s) {                                       \
+    return launch_wave_eval<gen::NAME##_wave::Wave>(a, s, 1);                                               \
+  }                                                                                                         \
+  const MechOps* mkpp_ops_##NAME() {                                                                        \
+    using W = gen::NAME##_wave::Wave;                                                                       \
+    static const MechOps ops = {#NAME, W::N, W::NUM_REACTIONS, W::NPH, W::NPH, 0,                          \
+                                W::SUPPORTS_RUNTIME_ENVIRONMENT, W::SPECIES,                                \
+                                &launch_wave_integrate<W>, &wave_eval_rates_##NAME, &wave_eval_jac_##NAME, \
+                                {F0, F1, F2, F3, F4}, nullptr, W::NG, W::NV,                               \
+                                &launch_wave_reduced<W>};                                                   \
+    return &ops;                                                                                            \
+  }                                                                                                         \
   }
 
 // Registers a generated wave mechanism: defines mkpp_ops_<name>().

@@ -19,7 +19,7 @@ import sys
 from . import plan as planmod
 from .model import BATCHED, BUNDLED, FORCE_WAVE, load_mechanism
 
-EXTRA = ("saprc99_mini", "ts1")
+EXTRA = ("saprc99_mini",)
 
 
 def main(argv=None):
@@ -43,7 +43,8 @@ def main(argv=None):
             from . import emit_unrolled
 
             outdir = os.path.join(root, "mkpp_amd", "csrc", "gen")
-            emit_unrolled.write(p, outdir)
+            if len(p["species"]) <= emit_unrolled.MAX_SPECIES:
+                emit_unrolled.write(p, outdir)
             nv = len(p["species"]) - sum(p["fixed"])
             if nv <= emit_hip.LANE_MAX_VARIABLE_SPECIES and name not in FORCE_WAVE:
                 emit_hip.write(mech, p, outdir)

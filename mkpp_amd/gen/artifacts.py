@@ -23,6 +23,8 @@ import hashlib
 import json
 import os
 
+from .model import NO_DIFF
+
 PLAN_BLOCK_OPERATIONS = 256
 
 
@@ -93,7 +95,7 @@ def manifest(p: dict, engine: str | None, plan_json: str, plan_md: str) -> dict:
     if engine is not None:
         artifacts += [{"kind": "shared_library", "file": f"lib/libmkpp_{name}.so"},
                       {"kind": "compiled_kernel_source", "file": f"csrc/gen/mech_{name}.hip"},
-                      {"kind": "adjoint_tlm_record", "differentiable": engine == "wave"}]
+                      {"kind": "adjoint_tlm_record", "differentiable": name not in NO_DIFF}]
     return {
         "mechanism": name,
         "aerosol_representation": "bulk",

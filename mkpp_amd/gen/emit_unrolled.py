@@ -27,6 +27,9 @@ import os
 
 # expressions per factorization function, as the reference chunks them
 CHUNK = 256
+# mechanisms up to this size get a header (ts1's 49,000 products are left out:
+# the host check would spend minutes compiling them)
+MAX_SPECIES = 100
 
 
 def symbolic(n: int, w_locations) -> dict:

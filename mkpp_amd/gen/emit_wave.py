@@ -22,7 +22,11 @@ from .emit_hip import VarMap
 from .model import Mechanism
 from .plan import LAW_IDS
 
+# reactant entries (distinct species) per flux group in the entry tables
 MAX_FACTORS = 3
+# factor slots (species repeated by power) per flux group: at least 3, more
+# where a group has more (ts1's HO2 + HO2 + H2O + M has four)
+MAX_FACTOR_SLOTS = 4
 
 
 # Terms per chunk of the two-level F sum (build_f_tables); a row of n terms
@@ -179,7 +183,10 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
 
     # flux factors: squared species first so the product equals the oracle's
     # y_a * (y_b * y_b) (IEEE multiplication commutes)
-    gfac = []  # [ng][3] y slots
+    nfac = max(3, max(sum(q for _, q in g["powers"]) for g in groups))
+    if nfac > MAX_FACTOR_SLOTS:
+        raise ValueError(f"more than {MAX_FACTOR_SLOTS} reactant factors in a flux group")
+    gfac = []  # [ng][nfac] y slots
     gpow_s, gpow_p, gnpow, gpow_k = [], [], [], []  # reactant entries in plan order
     law, photo, params = [], [], []
     for g in groups:
@@ -187,9 +194,9 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
         facs = []
         for s, q in sorted(ents, key=lambda e: -e[1]):
             facs += [yslot(s)] * q
-        if len(facs) > MAX_FACTORS:
-            raise ValueError("more than three reactant factors")
-        facs += [ONE] * (MAX_FACTORS - len(facs))
+        if len(ents) > MAX_FACTORS:
+            raise ValueError("more than three distinct reactants in a flux group")
+        facs += [ONE] * (nfac - len(facs))
         gfac.append(facs)
         ss = [yslot(s) for s, _ in ents] + [ONE] * (MAX_FACTORS - len(ents))
         pp = [q for _, q in ents] + [0] * (MAX_FACTORS - len(ents))
@@ -335,7 +342,7 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
         "unit": unit, "gpows_phys": gpows_phys, "gnuofs": gnuofs, "gnus": gnus, "gnuc": gnuc,
         "gdepofs": gdepofs, "gdep": gdep, "gnvofs": gnvofs, "gnvr": gnvr, "gnvc": gnvc, "spofs": spofs, "spi": spi,
         "name": p["name"], "n": n, "nv": nv, "nf": nfix, "ng": ng, "nph": p["num_runtime_parameters"],
-        "nb": nb, "span": span, "ny": nv + nfix + 1, "npart": npart, "one": ONE,
+        "nb": nb, "span": span, "ny": nv + nfix + 1, "npart": npart, "one": ONE, "nfac": nfac,
         "var_phys": vm.var_phys, "fixed_phys": vm.fixed_phys, "third_body": p["third_body"],
         "slot": slot, "rowof": rowof, "k0": k0, "R": R, "lcnt": lcnt, "lofs": lofs, "lrow": lrow, "ucnt": ucnt, "uofs": uofs, "ucol": ucol,
         "fmask": fmask, "bmask": bmask, "law": law, "photo": photo, "params": params, "gfac": gfac,
@@ -394,7 +401,7 @@ def emit(t: dict) -> str:
         _arr("uint16_t", "POFF", t["poff"]),
         _arr("double", "COEF", t["coefs"]),
         _arr("uint16_t", "FOFS", t["fofs"]),
-        _arr("uint8_t", "FG", t["fg"]),
+        _arr("uint16_t", "FG", t["fg"]),
         _arr("uint8_t", "FC", t["fc"]),
         _arr("uint16_t", "WADDR", t["waddr"]),
         _arr("uint8_t", "WDIAG", t["wdiag"]),
@@ -425,6 +432,10 @@ def emit(t: dict) -> str:
         f"  static constexpr int N = {t['n']}, NV = {t['nv']}, NF = {t['nf']}, NG = {t['ng']};",
         f"  static constexpr int NPH = {t['nph']}, NB = {t['nb']}, SPAN = {t['span']}, NY = {t['ny']};",
         f"  static constexpr int NPART = {t['npart']}, ONE = {t['one']}, NENTRIES = {t['nentries']};",
+        f"  static constexpr int NFAC = {t['nfac']};  // factor slots per flux group (GFAC)",
+        "  // wavefronts per SIMD the kernels are built for: 2 where the LDS block lets",
+        "  // 8 share a CU, else 1 (then spills go to AGPRs, not scratch)",
+        f"  static constexpr int WAVES_PER_EU = {2 if t['lds_doubles'] <= LDS_DOUBLES_8_WAVES else 1};",
         f"  static constexpr int NMWP = {t['nmwp']};  // reduction variant: passes over the W locations",
         f"  static constexpr int THIRD_BODY = {t['third_body']}, NUM_REACTIONS = {t['num_reactions']};",
         f"  static constexpr bool SUPPORTS_RUNTIME_ENVIRONMENT = {'true' if t['supports_runtime_environment'] else 'false'};",
@@ -439,14 +450,17 @@ def emit(t: dict) -> str:
 
 
 def emit_unit(t: dict) -> str:
+    from .model import NO_DIFF
+
     name = t["name"]
     sf = t["step_flops"]
     flops = ", ".join(f"{float(sf[m])!r}" for m in ("ros2", "ros3", "ros4", "rodas3", "rodas4"))
+    macro = "MKPP_WAVE_MECHANISM_NODIFF" if name in NO_DIFF else "MKPP_WAVE_MECHANISM"
     return (
         f"// Generated by mkpp_amd.gen.emit_wave: registers {name} (one cell per wavefront).\n"
         '#include "../adjoint.hpp"\n'
         f'#include "{name}_wcode.hpp"\n\n'
-        f"MKPP_WAVE_MECHANISM({name}, {flops})\n"
+        f"{macro}({name}, {flops})\n"
     )
 
 
@@ -560,13 +574,16 @@ def build_f_tables(t: dict, cidx: dict, zero_c: int, tf: int) -> dict:
     area, and the lane of each row then adds its chunks' partials in order.
 
     CF_TERM [pass][lane][tf/2] uint32: two terms a word (low half first), a
-      term = flux slot | coefficient index << 8; padding is the zero flux slot
-      NG with coefficient 0.0.
+      term = flux slot | coefficient index << FBITS (8, or 10 when NG >= 256);
+      padding is the zero flux slot NG with coefficient 0.0.
     CF_PART [bank][lane][maxcp] uint16: the row's partial slots, padded with
       the zero flux slot NG; maxc[bank] entries are read."""
     nv, ng, nb = t["nv"], t["ng"], t["nb"]
     coefs = t["coefs"]
-    assert tf % 2 == 0 and ng < 256 and len(coefs) <= 256
+    # a 16-bit term packs the flux slot in FBITS bits and the coefficient
+    # index above it: 8 + 8 while the zero slot NG fits a byte, else 10 + 6
+    fbits = 8 if ng < 256 else 10
+    assert tf % 2 == 0 and ng < (1 << fbits) and len(coefs) <= (1 << (16 - fbits))
     chunks = []  # (row, [(group, coef)])
     for v in range(nv):
         terms = [(t["fg"][k], coefs[t["fc"][k]]) for k in range(t["fofs"][v], t["fofs"][v + 1])]
@@ -579,7 +596,7 @@ def build_f_tables(t: dict, cidx: dict, zero_c: int, tf: int) -> dict:
         for lane in range(64):
             j = p * 64 + lane
             terms = chunks[j][1] if j < len(chunks) else []
-            code = [g | (cidx[c] << 8) for g, c in terms] + [ng | (zero_c << 8)] * (tf - len(terms))
+            code = [g | (cidx[c] << fbits) for g, c in terms] + [ng | (zero_c << fbits)] * (tf - len(terms))
             for q in range(0, tf, 2):
                 words.append(code[q] | (code[q + 1] << 16))
     parts = {v: [] for v in range(nv)}
@@ -598,7 +615,7 @@ def build_f_tables(t: dict, cidx: dict, zero_c: int, tf: int) -> dict:
             v = rowof.get(b * 64 + lane)
             ps = parts[v] if v is not None else []
             part += ps + [ng] * (maxcp - len(ps))
-    return {"tf": tf, "nfp": npass, "pbase": pbase, "fterm": words, "fpart": part, "fmaxc": maxc,
+    return {"tf": tf, "fbits": fbits, "nfp": npass, "pbase": pbase, "fterm": words, "fpart": part, "fmaxc": maxc,
             "fmaxcp": maxcp, "fslots": pbase + len(chunks)}
 
 
@@ -726,7 +743,7 @@ def build_code_tables(t: dict, tf: int = F_CHUNK_TERMS, tw: int = 2) -> dict:
                 else:
                     gterm.append(t["nv"] | (zero_c << 8))
         for lane in range(64):
-            ggrp.append(ps[lane][0] if lane < len(ps) else 255)
+            ggrp.append(ps[lane][0] if lane < len(ps) else 0xFFFF)
     assert t["nv"] < 256 and len(coefs) <= 256
     return {**ft, "tw": tw, "nwp": len(wpasses),
             "wterm": wterm, "wkey": waddr, **st,
@@ -769,10 +786,11 @@ def emit_code(t: dict, ct: dict) -> str:
     w(_arr_aligned("uint16_t", "CS_PART", ct["spart"], 8))
     w(_arr("uint16_t", "CS_FIX", ct["sfix"]))
     w(_arr("uint16_t", "CG_TERM", ct["gterm"]))
-    w(_arr("uint8_t", "CG_GRP", ct["ggrp"]))
+    w(_arr("uint16_t", "CG_GRP", ct["ggrp"]))
     w("")
     w("struct Code {")
     w(f"  static constexpr int TF = {ct['tf']}, NFP = {ct['nfp']}, TW = {ct['tw']}, NWP = {ct['nwp']};")
+    w(f"  static constexpr int FBITS = {ct['fbits']};  // flux-slot bits of a 16-bit F term (coefficient above)")
     w(f"  // F partial slots scr[PBASE, PBASE + chunks); a row reads FMAXC[bank] of its FMAXCP-wide list")
     w(f"  static constexpr int PBASE = {ct['pbase']}, FSLOTS = {ct['fslots']}, FMAXCP = {ct['fmaxcp']};")
     w(f"  static constexpr int FMAXC[{nb}] = {{{', '.join(str(m) for m in ct['fmaxc'])}}};")

@@ -559,7 +559,12 @@ def load_mechanism(name_or_path: str) -> Mechanism:
 # decay (X -> nothing at a per-cell rate) is the scalar test equation of the
 # step-size controller's known-answer tests; decay_wave is the same mechanism
 # built on the wave-per-cell engine, so that those tests reach both engines.
-BUNDLED = ("chapman", "small_strato", "carbon", "saprc99", "saprcnov", "decay", "decay_wave")
+# ts1 (210 species) runs on the wave engine with four rows per lane.
+BUNDLED = ("chapman", "small_strato", "carbon", "saprc99", "saprcnov", "decay", "decay_wave", "ts1")
+# Wave mechanisms registered without the differentiation kernels (their
+# record, adjoint and tangent-linear instantiations would add minutes to every
+# build of a 49,000-product factorization).
+NO_DIFF = ("ts1",)
 # Mechanisms built on the wave-per-cell engine whatever their size.
 FORCE_WAVE = ("decay_wave",)
 # Mechanisms of the persistent batch kernel (each tableau's kernel carries every

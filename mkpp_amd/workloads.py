@@ -14,6 +14,8 @@ tolerances the configuration integrates with.
                     half of the cells at night
 * carbon_mixed    - carbon, Rodas3 adaptive, T-dependent rate constants
 * saprcnov_urban  - saprcnov with the saprc99 urban setup and matched photolysis rates
+* ts1_scenario    - ts1 (mol m-3): the reference runner's binding data, Ros3 at the
+                    measurement scenario's accuracy controls, 300 s + 300 s
 * batch_mixed     - the batching workload: saprc99 (NOx, VOCs 1e8-1e13, half night),
                     carbon and small_strato (half night) cells, Rodas3 adaptive
                     at rtol 1e-3 for all three in one launch
@@ -222,3 +224,40 @@ def saprc99_diff(num_cells: int = 65536, seed: int = 5) -> dict:
     w["target_initial"] = np.maximum(target, 0.0)
     w["loss_species"] = [mech.names.index("O3"), mech.names.index("NO2")]
     return w
+
+
+def ts1_scenario(num_cells: int = 1024, seed: int | None = 7, spread: float = 0.1) -> dict:
+    """The ts1 measurement scenario (mol m-3; 210 species, 547 rate-parameter
+    slots): the initial state, rate parameters, temperature and air density the
+    reference's solver-comparison runner binds (runners/mkpp/main.cpp:206-215,
+    326-342; data in mechanisms/ts1/scenario.json), Ros3, the scenario's
+    accuracy controls (rtol 1e-6, atol per species).  ``rate_parameters`` is
+    the first segment's; ``segments`` lists (seconds, photolysis scale of slots
+    0..122) for the runner's 300 s + 300 s schedule.
+
+    The runner gives every cell the same state.  With ``seed`` set, each cell's
+    initial state and photolysis rates are scaled by exp(spread * N(0, 1)) so
+    that the cells differ; seed None keeps them identical."""
+    import json
+    import os
+
+    d = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "mechanisms", "ts1", "scenario.json")))
+    y0 = np.asarray(d["initial_state"], dtype=np.float64)
+    rp0 = np.asarray(d["reaction_parameters"], dtype=np.float64)
+    conc = np.repeat(y0[:, None], num_cells, axis=1)
+    rp = np.repeat(rp0[:, None], num_cells, axis=1)
+    photo = np.asarray(d["photolysis_slots"])
+    if seed is not None:
+        rng = np.random.default_rng(seed)
+        fixed = d["species"].index("M")
+        scale = np.exp(spread * rng.standard_normal(conc.shape))
+        scale[fixed] = 1.0
+        conc = conc * scale
+        rp[photo] = rp[photo] * np.exp(spread * rng.standard_normal((len(photo), num_cells)))
+    T = np.full(num_cells, float(d["temperature"]))
+    rho = np.full(num_cells, float(d["air_density"]))
+    return {"mechanism": "ts1", "conc": np.ascontiguousarray(conc), "rate_parameters": np.ascontiguousarray(rp),
+            "temp": T, "pressure": np.full(num_cells, 101319.9), "air_density": rho, "method": "ros3",
+            "dt": 300.0, "steps": 1, "seed": 1.0e-6, "atol": np.asarray(d["atol"]), "rtol": float(d["rtol"]),
+            "photolysis_slots": photo,
+            "segments": [(s["seconds"], s["photolysis_scale"]) for s in d["segments"]]}

@@ -200,6 +200,6 @@ def test_dropin_tool_rejects_unknown_mechanism(tmp_path):
     from mkpp_amd import dropin
 
     with pytest.raises(ValueError):
-        dropin.build("ts1", str(tmp_path / "x"))
+        dropin.build("saprc99_mini", str(tmp_path / "x"))  # planned, not compiled in
     with pytest.raises(ValueError):
         dropin.build("saprc99", str(tmp_path / "y"), solver="ros9")

@@ -38,7 +38,7 @@ def _plan(name):
         return json.load(f)
 
 
-@pytest.mark.parametrize("name", list(BUNDLED) + ["saprc99_mini", "ts1"])
+@pytest.mark.parametrize("name", list(BUNDLED) + ["saprc99_mini"])
 def test_structure_equals_plan(name):
     p = _plan(name)
     sym = emit_unrolled.symbolic(len(p["species"]), p["w_locations"])
@@ -82,7 +82,11 @@ def _host_lib(name, tmp_path):
     return lib, lambda a: a.ctypes.data_as(dp)
 
 
-@pytest.mark.parametrize("name", BUNDLED)
+# the mechanisms with a generated header (ts1 has none)
+WITH_HEADER = [m for m in BUNDLED if len(_plan(m)["species"]) <= emit_unrolled.MAX_SPECIES]
+
+
+@pytest.mark.parametrize("name", WITH_HEADER)
 def test_host_build_equals_oracle_plan(name, tmp_path):
     p = _plan(name)
     n = len(p["species"])
