@@ -55,6 +55,8 @@ def main():
                     help="skip the reduction-variant leg (saprc99, importance threshold 1e-3)")
     ap.add_argument("--no-saprcnov", dest="saprcnov", action="store_false",
                     help="skip the second wave-engine mechanism reported as saprcnov")
+    ap.add_argument("--no-ts1", dest="ts1", action="store_false",
+                    help="skip the ts1 runner-schedule leg (210 species)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB elementwise copy (x * 1.0) three times: the known-size kernel "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -157,6 +159,8 @@ def main():
     progress("saprcnov leg done")
     reduced = run_reduced(np, w, ncell, args.cpu_sample, args.cpu_threads) if args.reduced else None
     progress("reduction-variant leg done")
+    ts1 = run_ts1(np, 1024, 16 if args.cpu_sample else 0, args.cpu_threads) if args.ts1 else None
+    progress("ts1 leg done")
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "saprc99_rodas3_262144_traffic.json")
@@ -203,6 +207,7 @@ def main():
         "adjoint": adjoint,
         "saprcnov": saprcnov,
         "reduced": reduced,
+        "ts1": ts1,
         "configs": configs,
     }
     if cpu:
@@ -254,14 +259,39 @@ def cpu_leg(spec: dict) -> dict:
     w = getattr(workloads, spec["workload"])(spec["cells"])
     m = spec["sample"]
     o = Oracle(w["mechanism"])
-    jv = np.ascontiguousarray(w["rate_parameters"][:, :m]) if o.nphoto else None
+    # (seconds, photolysis scale) per segment: the ts1 runner's schedule
+    segments = w.get("segments", [(w["dt"], 1.0)])
+    jvs = []
+    for _, scale in segments:
+        jv = None
+        if o.nphoto:
+            jv = np.ascontiguousarray(w["rate_parameters"][:, :m]).copy()
+            if scale != 1.0:
+                jv[w["photolysis_slots"]] *= scale
+        jvs.append(jv)
+
+    def run(c, lo, hi, threads):
+        for (sec, _), jv in zip(segments, jvs):
+            o.integrate(c, sec, w["steps"], w["method"], temp=w["temp"][lo:hi], air_density=w["air_density"][lo:hi],
+                        jvals=None if jv is None else np.ascontiguousarray(jv[:, lo:hi]), atol=w["atol"],
+                        rtol=w["rtol"], seed=w["seed"], hoist=spec.get("hoist", False),
+                        max_attempts=spec.get("max_attempts", 0), threads=threads)
+
     times = []
     for r in range(1 + spec.get("reps", 3)):
         c = np.ascontiguousarray(w["conc"][:, :m]).copy()
-        t = time.perf_counter()
-        o.integrate(c, w["dt"], w["steps"], w["method"], temp=w["temp"][:m], air_density=w["air_density"][:m],
-                    jvals=jv, atol=w["atol"], rtol=w["rtol"], seed=w["seed"], hoist=spec.get("hoist", False),
-                    max_attempts=spec.get("max_attempts", 0), threads=spec["threads"])
+        if spec.get("cell_pool"):
+            # one cell per task on a pool of spec["threads"] host threads (the
+            # oracle's OpenMP chunks of 64 cells would put a small sample on one)
+            from concurrent.futures import ThreadPoolExecutor
+
+            cols = [np.ascontiguousarray(c[:, i:i + 1]) for i in range(m)]
+            t = time.perf_counter()
+            with ThreadPoolExecutor(max_workers=spec["threads"]) as ex:
+                list(ex.map(lambda i: run(cols[i], i, i + 1, 1), range(m)))
+        else:
+            t = time.perf_counter()
+            run(c, 0, m, spec["threads"])
         if r > 0:
             times.append(time.perf_counter() - t)
     med = float(np.median(times))
@@ -272,6 +302,12 @@ def _cpu_child(spec: dict, threads: int) -> dict:
     import subprocess
 
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    if spec.get("cell_pool"):
+        # host threads of a pool, one oracle call each: no OpenMP binding (a
+        # bound initial thread would pass its one-core mask to every pool thread)
+        env.pop("OMP_PROC_BIND")
+        env.pop("OMP_PLACES")
+        env["OMP_NUM_THREADS"] = "1"
     spec = dict(spec, threads=threads)
     r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg", json.dumps(spec)], env=env,
                        capture_output=True, text=True, timeout=600)
@@ -292,8 +328,9 @@ def cpu_baseline(spec: dict, threads: int = 0, single_thread_sample: int = 0, ho
     main = _cpu_child(spec, threads)
     out = {"value": main["cells_per_s"], "unit": "grid-cells/s", "cores": threads, "kind": "port",
            "cpu_model": model, "physical_cores_visible": phys, "logical_cpus_visible": logical,
-           "omp": "OMP_PROC_BIND=close OMP_PLACES=cores, schedule(static, 64) over cells",
-           "timing": "1 warm-up + 3 timed runs, median; state copied outside the timer",
+           "omp": ("a pool of host threads, one cell per task, one oracle thread each" if spec.get("cell_pool")
+                   else "OMP_PROC_BIND=close OMP_PLACES=cores, schedule(static, 64) over cells"),
+           "timing": f"1 warm-up + {spec.get('reps', 3)} timed run(s), median; state copied outside the timer",
            "sample": f"first {spec['sample']} of the {spec['cells']} cells of {spec['workload']}, "
                      f"median {main['median_s']:.2f} s per run",
            "runs_s": main["seconds"]}
@@ -422,6 +459,69 @@ def run_saprcnov(np, cells, cpu_sample, cpu_threads, reps=2, max_attempts=5000):
             "attempted_steps_mean": float(attempted.mean()), "max_attempts": max_attempts,
             "cells_at_attempt_limit": int((st != 0).sum()),
             "fp64_tflops": achieved / 1e12, "roofline_frac": achieved / PEAK_FP64, "cpu_baseline": cpu}
+
+
+def run_ts1(np, cells, cpu_sample, cpu_threads):
+    """ts1 (210 species, 533 flux groups, 4,802 LU entries, mol m-3) on the
+    reference runner's schedule (runners/mkpp/main.cpp:326-342): 300 s with
+    the binding's rate parameters, then 300 s with photolysis slots 0..122
+    scaled by 1.05, Ros3 at the measurement scenario's accuracy controls
+    (rtol 1e-6, atol per species), cells scaled by exp(0.1 N(0, 1)) so they
+    differ.  Device-resident; one timed pass of both segments after a warm-up
+    on 64 cells, HIP events on the launch stream.  The runner counts
+    cell_steps = cells x 2 segments."""
+    import torch
+
+    from mkpp_amd import native, workloads
+
+    w = workloads.ts1_scenario(cells)
+    plan = json.load(open(os.path.join(ROOT, "mkpp_amd", "plans", "ts1.json")))
+    step_flops = plan["step_flops"]["ros3"]
+    dev = torch.device("cuda:0")
+    n = w["conc"].shape[0]
+    init = torch.tensor(w["conc"], device=dev).contiguous()
+    T = torch.tensor(w["temp"], device=dev)
+    rho = torch.tensor(w["air_density"], device=dev)
+    at = torch.tensor(w["atol"], device=dev)
+    rt = torch.full((n,), w["rtol"], dtype=torch.float64, device=dev)
+    rps = []
+    for _, scale in w["segments"]:
+        rp = w["rate_parameters"].copy()
+        rp[w["photolysis_slots"]] *= scale
+        rps.append(torch.tensor(rp, device=dev).contiguous())
+    st = torch.zeros(cells, dtype=torch.int32, device=dev)
+    att = torch.zeros(cells, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def schedule(buf, T, rho, rps, st, att):
+        for (sec, _), rp in zip(w["segments"], rps):
+            native.device_integrate("ts1", buf, sec, 1, "ros3", T, rho, rp, at, rt, w["seed"], status=st,
+                                    attempted=att, stream=stream)
+
+    m = 64  # warm-up cells
+    schedule(init[:, :m].contiguous(), T[:m].contiguous(), rho[:m].contiguous(),
+             [rp[:, :m].contiguous() for rp in rps], st[:m].contiguous(), att[:m].contiguous())
+    buf = init.clone()
+    att.zero_()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record(stream)
+    schedule(buf, T, rho, rps, st, att)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1])
+    attempted = att.cpu().numpy().astype(np.float64)
+    achieved = float(attempted.sum()) * step_flops / (ms / 1e3)
+    cpu = None
+    if cpu_sample > 0:
+        cpu = cpu_baseline({"workload": "ts1_scenario", "cells": cells, "sample": min(cpu_sample, cells),
+                            "cell_pool": True, "reps": 1}, cpu_threads)
+    return {"workload": f"ts1 runner schedule (300 s + 300 s, photolysis x1.05 in the second), Ros3 rtol 1e-6 "
+                        f"atol per species, {cells} cells", "value": cells / (ms / 1e3),
+            "unit": "grid-cells/s (whole schedule)", "runner_cell_steps_per_s": 2 * cells / (ms / 1e3),
+            "ms": ms, "attempted_steps_mean": float(attempted.mean()), "status_nonzero": int((st != 0).sum()),
+            "fp64_tflops": achieved / 1e12, "roofline_frac": achieved / PEAK_FP64,
+            "flops_per_attempted_step": step_flops, "cpu_baseline": cpu}
 
 
 def run_reduced(np, w, cells, cpu_sample, cpu_threads, threshold=1e-3, reps=2):
