@@ -210,8 +210,10 @@ int mkpp_amd_batch_capacity(int method, int* per_cu, int* num_cus);
  * a forcing shorter than the mechanism's runtime parameters, or a scalar
  * temperature or density that is not positive and finite where it is used;
  * MKPP_ERROR_CONVERGENCE if a cell reached the attempt limit.
- * rh, clw, cell_humidity and cell_cloud_water are accepted and validated; no
- * bundled mechanism reads them. */
+ * clw and cell_cloud_water (cloud liquid water) gate the cloud-gated
+ * heterogeneous reactions (gocart's SO2 + H2O2); rh and cell_humidity are
+ * accepted and validated, and no bundled mechanism reads them (the
+ * reference's generated code leaves RH unused as well). */
 typedef struct mkpp_amd_execution_params {
   double dt;
   int steps;

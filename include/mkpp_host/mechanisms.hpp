@@ -43,6 +43,10 @@
 #include "ts1.hpp"
 #define MKPP_HOST_HAS_TS1 1
 #endif
+#if __has_include("gocart.hpp")
+#include "gocart.hpp"
+#define MKPP_HOST_HAS_GOCART 1
+#endif
 
 namespace mkpp::host::detail {
 
@@ -121,6 +125,16 @@ inline bool generated_options(const std::string& name, SolverOptions& o) {
 #ifdef MKPP_HOST_HAS_TS1
   if (name == "ts1") {
     using K = ::mkpp::generated::ts1::SolverKernels<>;
+    o.method = K::METHOD;
+    o.atol = K::atol;
+    o.rtol = K::rtol;
+    o.initial_step_seed = K{}.initial_step_seed;
+    return true;
+  }
+#endif
+#ifdef MKPP_HOST_HAS_GOCART
+  if (name == "gocart") {
+    using K = ::mkpp::generated::gocart::SolverKernels<>;
     o.method = K::METHOD;
     o.atol = K::atol;
     o.rtol = K::rtol;

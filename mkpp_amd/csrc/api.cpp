@@ -64,6 +64,7 @@ struct mkpp_handle_s {
   hipStream_t stream = nullptr;
   size_t cap_cells = 0;
   double *d_conc = nullptr, *d_temp = nullptr, *d_rho = nullptr, *d_par = nullptr, *d_tol = nullptr, *d_out = nullptr;
+  double* d_clw = nullptr;  // staged cloud liquid water (when a cloud-water pointer is set)
   size_t cap_out = 0;
   int *d_status = nullptr, *d_acc = nullptr, *d_att = nullptr;
   std::vector<int> status, accepted, attempted;
@@ -71,7 +72,7 @@ struct mkpp_handle_s {
 
   void set_error(const std::string& s) { last_error = s; }
   ~mkpp_handle_s() {
-    for (void* p : {(void*)d_conc, (void*)d_temp, (void*)d_rho, (void*)d_par, (void*)d_tol, (void*)d_out,
+    for (void* p : {(void*)d_conc, (void*)d_temp, (void*)d_rho, (void*)d_par, (void*)d_tol, (void*)d_out, (void*)d_clw,
                     (void*)d_status, (void*)d_acc, (void*)d_att})
       (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
@@ -133,6 +134,7 @@ int ensure_device(mkpp_handle_s* h) {
   if ((rc = grow(h, &h->d_conc, n * N))) return rc;
   if ((rc = grow(h, &h->d_temp, n))) return rc;
   if ((rc = grow(h, &h->d_rho, n))) return rc;
+  if ((rc = grow(h, &h->d_clw, n))) return rc;
   if ((rc = grow(h, &h->d_par, n * P))) return rc;
   if ((rc = grow(h, &h->d_tol, 2 * N))) return rc;
   if ((rc = grow(h, &h->d_status, n))) return rc;
@@ -150,6 +152,8 @@ int upload_state(mkpp_handle_s* h) {
   HIPCHK(h, hipMemcpyAsync(h->d_rho, h->air_density, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
   if (P > 0 && h->params)
     HIPCHK(h, hipMemcpyAsync(h->d_par, h->params, n * P * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  if (h->cloud_water)
+    HIPCHK(h, hipMemcpyAsync(h->d_clw, h->cloud_water, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
   return MKPP_SUCCESS;
 }
 
@@ -396,6 +400,7 @@ int mkpp_amd_integrate_steps(mkpp_handle_t handle, double dt, int steps) {
   a.accepted = h->d_acc;
   a.attempted = h->d_att;
   a.max_attempts = h->max_attempts;
+  a.cloud_water = h->cloud_water ? h->d_clw : nullptr;
   if (h->ops->integrate(a, h->stream) != 0) {
     h->set_error("FATAL ERROR: integrate kernel launch failed");
     return MKPP_ERROR_INVALID_STATE;
@@ -494,6 +499,8 @@ int mkpp_amd_execute(mkpp_handle_t handle, double* conc, long num_cells, long st
   a.accepted = h->d_acc;
   a.attempted = h->d_att;
   a.max_attempts = h->max_attempts;
+  a.cloud_water = p.cell_cloud_water;
+  a.clw_default = p.clw;
   if (h->ops->integrate(a, s) != 0) {
     h->set_error("FATAL ERROR: integrate kernel launch failed");
     return MKPP_ERROR_INVALID_STATE;
@@ -559,6 +566,7 @@ static int evaluate(mkpp_handle_t handle, double* out, bool jac) {
   e.jstride = (long)n;
   e.out = h->d_out;
   e.out_stride = (long)n;
+  e.cloud_water = h->cloud_water ? h->d_clw : nullptr;
   if ((jac ? h->ops->eval_jacobian(e, h->stream) : h->ops->eval_rates(e, h->stream)) != 0) {
     h->set_error("FATAL ERROR: evaluation kernel launch failed");
     return MKPP_ERROR_INVALID_STATE;

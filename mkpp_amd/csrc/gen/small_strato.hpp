@@ -13,6 +13,9 @@ struct Mech {
   static constexpr int NPHOTO = 4;
   static constexpr int NUM_RUNTIME_PARAMETERS = 4;
   static constexpr int NG = 10;
+  // rate-constant slots: the flux groups, then Kp(T) of each equilibrium block
+  static constexpr int NK = 10;
+  static constexpr bool USES_CLOUD_WATER = false;
   static constexpr int NLU = 29;
   static constexpr int NJ = 28;
   static constexpr int NUM_REACTIONS = 10;
@@ -49,10 +52,12 @@ struct Mech {
     return t[v];
   }
 
-  // k_r and dk_r/d[M] per flux group (src/mkpp/lowering.py:199-313)
+  // k_r and dk_r/d[M] per flux group (src/mkpp/lowering.py:199-313); clw: the
+  // cell's cloud liquid water (cloud-gated heterogeneous reactions)
   static __device__ __forceinline__ void rate_constants(double T, double M, const double* __restrict__ jv,
-                                                        double* __restrict__ k, double* __restrict__ dk) {
-    (void)T; (void)M; (void)jv;
+                                                        double clw, double* __restrict__ k,
+                                                        double* __restrict__ dk) {
+    (void)T; (void)M; (void)jv; (void)clw;
     k[0] = jv[0];
     dk[0] = 0.0;
     k[1] = (8.018e-17 * 1.0);

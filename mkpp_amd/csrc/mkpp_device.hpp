@@ -57,6 +57,8 @@ struct DeviceArgs {
   int* attempted;      // [ncells] or null
   long max_attempts;   // per integrate() call; <= 0 means unbounded
   double importance_threshold;  // reduction variant only: species below it are frozen
+  const double* cloud_water = nullptr;  // [ncells] cloud liquid water, or null (clw_default)
+  double clw_default = 0.0;
 };
 
 // Differentiation arguments (checkpointed forward, frozen-step replay, adjoint).
@@ -439,7 +441,7 @@ __device__ __forceinline__ bool all_finite(const double* v, int n) {
 template <class M, class Tab, bool REDUCED = false>
 __device__ __forceinline__ void integrate_lane_cell(const DeviceArgs& a, long cell) {
   double x[M::NV], yf[M::NF], jv[M::NPHOTO];
-  double k[M::NG], dk[M::NG];
+  double k[M::NK], dk[M::NK];
 #pragma unroll
   for (int v = 0; v < M::NV; ++v) {
     x[v] = a.conc[cell + a.stride * M::var_phys(v)];
@@ -461,7 +463,7 @@ __device__ __forceinline__ void integrate_lane_cell(const DeviceArgs& a, long ce
     a.status[cell] = ERR_NON_FINITE_STATE;
     return;
   }
-  M::rate_constants(T, Mtb, jv, k, dk);
+  M::rate_constants(T, Mtb, jv, a.cloud_water ? a.cloud_water[cell] : a.clw_default, k, dk);
   int acc = 0, att = 0;
   for (int step = 0; step < a.steps && status == SUCCESS; ++step) {
     if constexpr (REDUCED)
@@ -504,6 +506,8 @@ struct EvalArgs {
   double rho_default = 2.4476e19;
   double* out;  // rates: [ncells x N]; jacobian: [ncells x N x N], LayoutLeft
   long out_stride;
+  const double* cloud_water = nullptr;  // [ncells] cloud liquid water, or null (clw_default)
+  double clw_default = 0.0;
 };
 
 template <class M>
@@ -517,14 +521,14 @@ __device__ __forceinline__ void eval_setup(const EvalArgs& a, long cell, double 
   const double T = a.temp ? a.temp[cell] : a.temp_default;
   const double rho = a.air_density ? a.air_density[cell] : a.rho_default;
   const double Mtb = M::THIRD_BODY >= 0 ? y[M::THIRD_BODY >= 0 ? M::THIRD_BODY : 0] : rho;
-  M::rate_constants(T, Mtb, jv, k, dk);
+  M::rate_constants(T, Mtb, jv, a.cloud_water ? a.cloud_water[cell] : a.clw_default, k, dk);
 }
 
 template <class M>
 __global__ void __launch_bounds__(64) eval_rates_kernel(EvalArgs a) {
   const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= a.ncells) return;
-  double y[M::N], k[M::NG], dk[M::NG], f[M::N];
+  double y[M::N], k[M::NK], dk[M::NK], f[M::N];
   eval_setup<M>(a, cell, y, k, dk);
   M::rates_full(y, k, f);
 #pragma unroll
@@ -535,7 +539,7 @@ template <class M>
 __global__ void __launch_bounds__(64) eval_jacobian_kernel(EvalArgs a) {
   const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= a.ncells) return;
-  double y[M::N], k[M::NG], dk[M::NG];
+  double y[M::N], k[M::NK], dk[M::NK];
   eval_setup<M>(a, cell, y, k, dk);
   M::jac_full(y, k, dk, a.out + cell, a.out_stride);
 }

@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(64) integrate_unrolled_kernel(DeviceArgs a) {
   const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= a.ncells) return;
   static_assert(U::N == M::N, "unrolled and lane tables describe different species counts");
-  double y[M::N], jv[M::NPHOTO], k[M::NG], dk[M::NG];
+  double y[M::N], jv[M::NPHOTO], k[M::NK], dk[M::NK];
   for (int i = 0; i < M::N; ++i) y[i] = a.conc[cell + a.stride * i];
   for (int p = 0; p < M::NPHOTO; ++p)
     jv[p] = (a.jvals != nullptr && p < M::NUM_RUNTIME_PARAMETERS) ? a.jvals[cell * a.jcstride + a.jstride * p] : 0.0;
@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(64) integrate_unrolled_kernel(DeviceArgs a) {
     a.status[cell] = ERR_NON_FINITE_STATE;
     return;
   }
-  M::rate_constants(T, Mtb, jv, k, dk);
+  M::rate_constants(T, Mtb, jv, a.cloud_water ? a.cloud_water[cell] : a.clw_default, k, dk);
   int status = SUCCESS, acc = 0, att = 0;
   for (int step = 0; step < a.steps && status == SUCCESS; ++step)
     status = integrate_cell_unrolled<M, U, Tab>(y, k, dk, a.dt, a.seed, a.atol, a.rtol, a.max_attempts, acc, att);

@@ -17,7 +17,7 @@ import os
 import sys
 
 from . import plan as planmod
-from .model import BATCHED, BUNDLED, FORCE_WAVE, load_mechanism
+from .model import BATCHED, BUNDLED, FORCE_WAVE, NO_DIFF, load_mechanism
 
 EXTRA = ("saprc99_mini",)
 
@@ -49,7 +49,8 @@ def main(argv=None):
             if nv <= emit_hip.LANE_MAX_VARIABLE_SPECIES and name not in FORCE_WAVE:
                 emit_hip.write(mech, p, outdir)
                 # the wave engine's tables serve the lane mechanism's differentiation
-                emit_wave.write_headers(mech, p, outdir)
+                if name not in NO_DIFF:
+                    emit_wave.write_headers(mech, p, outdir)
                 engines[name] = (name, "lane", 0)
             else:
                 emit_wave.write(mech, p, outdir)

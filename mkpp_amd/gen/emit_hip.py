@@ -105,7 +105,13 @@ def _k_code(g: dict, T="T", M="M") -> tuple[list[str], str, str]:
     if law == "CONSTANT":
         return [], P["A"], "0.0"
     if law == "HETEROGENEOUS":
-        return [], f"(0.25 * {P['gamma']})", "0.0"
+        if num.get("gate_width", 0.0) == 0.0:
+            return [], f"(0.25 * {P['gamma']})", "0.0"
+        # cloud-water gate 1/2 (1 + u / sqrt(1 + u^2)), u = (CLW - threshold) / width
+        # (for u < 0 as 1 / (2 q (q - u)), q = sqrt(1 + u^2), which does not cancel)
+        st = [f"const double u = (clw - {P['gate_threshold']}) / {P['gate_width']};",
+              "const double q = sqrt(1.0 + u * u);"]
+        return st, f"(0.25 * {P['gamma']} * (u >= 0.0 ? 0.5 * (1.0 + u / q) : 0.5 / (q * (q - u))))", "0.0"
     if law == "TROE":
         st = [
             f"const double k0 = {P['A0']} * exp({P['C0']} / {T}) * pow({T} / 300.0, {P['B0']});",
@@ -211,6 +217,8 @@ def emit(mech: Mechanism, p: dict) -> str:
     w("// Do not edit: regenerate with `python -m mkpp_amd.gen`.")
     w("#pragma once")
     w('#include "../mkpp_device.hpp"')
+    if p.get("equilibria"):
+        w('#include "../equilibrium.hpp"')
     w("")
     w(f"namespace mkpp {{ namespace gen {{ namespace {name} {{")
     w("")
@@ -222,6 +230,10 @@ def emit(mech: Mechanism, p: dict) -> str:
     w(f"  static constexpr int NPHOTO = {max(p['num_runtime_parameters'], 1)};")
     w(f"  static constexpr int NUM_RUNTIME_PARAMETERS = {p['num_runtime_parameters']};")
     w(f"  static constexpr int NG = {max(len(groups), 1)};")
+    eqs = p.get("equilibria", [])
+    w(f"  // rate-constant slots: the flux groups, then Kp(T) of each equilibrium block")
+    w(f"  static constexpr int NK = {max(len(groups), 1) + len(eqs)};")
+    w(f"  static constexpr bool USES_CLOUD_WATER = {'true' if p.get('uses_cloud_water') else 'false'};")
     w(f"  static constexpr int NLU = {vm.nlu};")
     w(f"  static constexpr int NJ = {len(p['j_entries'])};")
     w(f"  static constexpr int NUM_REACTIONS = {p['num_reactions']};")
@@ -258,10 +270,12 @@ def emit(mech: Mechanism, p: dict) -> str:
     w("  }")
     w("")
     # rate constants
-    w("  // k_r and dk_r/d[M] per flux group (src/mkpp/lowering.py:199-313)")
+    w("  // k_r and dk_r/d[M] per flux group (src/mkpp/lowering.py:199-313); clw: the")
+    w("  // cell's cloud liquid water (cloud-gated heterogeneous reactions)")
     w("  static __device__ __forceinline__ void rate_constants(double T, double M, const double* __restrict__ jv,")
-    w("                                                        double* __restrict__ k, double* __restrict__ dk) {")
-    w("    (void)T; (void)M; (void)jv;")
+    w("                                                        double clw, double* __restrict__ k,")
+    w("                                                        double* __restrict__ dk) {")
+    w("    (void)T; (void)M; (void)jv; (void)clw;")
     for gi, g in enumerate(groups):
         st, kx, dkx = _k_code(g)
         if st:
@@ -274,8 +288,57 @@ def emit(mech: Mechanism, p: dict) -> str:
         else:
             w(f"    k[{gi}] = {kx};")
             w(f"    dk[{gi}] = {dkx};")
+    for b in range(len(eqs)):
+        w(f"    k[{max(len(groups), 1) + b}] = nh4no3_kp(T);")
+        w(f"    dk[{max(len(groups), 1) + b}] = 0.0;")
     w("  }")
     w("")
+    # equilibrium blocks: per output species (physical) its block, output slot
+    # and the element of every column it depends on
+    from .model import PARTITION_MODELS
+
+    eq_row = {}  # physical output species -> (block, output slot)
+    eq_col = []  # per block: physical species -> element slot
+    for b, e in enumerate(eqs):
+        names, _ = PARTITION_MODELS[e["system"]]
+        eq_col.append({sp: el for el, nm in enumerate(names) for sp in e["elements"][nm]})
+        for kk, sp in enumerate(e["outputs"]):
+            if not p["fixed"][sp]:
+                eq_row[sp] = (b, kk)
+
+    def eq_setup(deriv: bool, ref) -> list:
+        """Statements evaluating each block's partition (values, and with deriv
+        the derivatives in the element totals) from the state via `ref`."""
+        out = []
+        for b, e in enumerate(eqs):
+            names, _ = PARTITION_MODELS[e["system"]]
+            tots = []
+            for nm in names:
+                t = "0.0"
+                for sp in e["elements"][nm]:
+                    t = f"({t} + {ref(sp)})"
+                tots.append(t)
+            out.append(f"    double eqv{b}[6], eqd{b}[6][3], eqdd{b}[6];")
+            out.append(f"    nh4_no3_so4_partition<{'true' if deriv else 'false'}>({tots[0]}, {tots[1]}, {tots[2]}, "
+                       f"k[{max(len(groups), 1) + b}], 1.0 / {_lit(e['width'])}, eqv{b}, eqd{b}, eqdd{b});")
+        return out
+
+    def eq_f_term(sp: int, ref) -> str:
+        b, kk = eq_row[sp]
+        return f"({_lit(eqs[b]['tau_inv'])} * (eqv{b}[{kk}] - {ref(sp)}))"
+
+    def eq_j_terms(i: int, j: int, expr: str) -> str:
+        """The block's tau_inv d(eq_i - y_i)/dy_j of row i added after the
+        mass-action sum, as the oracle does (the diagonal from eqdd)."""
+        if i not in eq_row:
+            return expr
+        b, kk = eq_row[i]
+        el = eq_col[b].get(j)
+        if el is not None:
+            d = f"eqdd{b}[{kk}]" if i == j else f"eqd{b}[{kk}][{el}]"
+            expr = f"({expr} + ({_lit(eqs[b]['tau_inv'])} * {d}))"
+        return expr
+
     # rates (variable, logical order)
     contrib = {i: [] for i in range(n)}  # physical species -> [(group, coef)]
     for gi, g in enumerate(groups):
@@ -288,14 +351,18 @@ def emit(mech: Mechanism, p: dict) -> str:
     used = sorted({gi for i in vm.var_phys for gi, _ in contrib[i]})
     for gi in used:
         w(f"    const double fl{gi} = {_flux_expr(vm, groups[gi], f'k[{gi}]')};")
+    for st in eq_setup(False, vm.yref):
+        w(st)
     for v, ph in enumerate(vm.var_phys):
         terms = contrib[ph]
-        if not terms:
+        if not terms and ph not in eq_row:
             w(f"    f[{v}] = 0.0;")
             continue
         expr = "0.0"
         for gi, c in terms:
             expr = f"({expr} + {_coef_times(c, f'fl{gi}')})"
+        if ph in eq_row:
+            expr = f"({expr} + {eq_f_term(ph, vm.yref)})"
         w(f"    f[{v}] = {expr};")
     w("  }")
     w("")
@@ -319,14 +386,19 @@ def emit(mech: Mechanism, p: dict) -> str:
                 need_part.add((gi, j))
     for gi, j in sorted(need_part):
         w(f"    const double d{gi}_{j} = {_partial_expr(vm, groups[gi], j, f'k[{gi}]', f'dk[{gi}]', tb)};")
+    for st in eq_setup(True, vm.yref):
+        w(st)
+    wset_log = {tuple(x) for x in vm.w_locs}
     for s, (li, lj) in enumerate(vm.lu_locs):
         pi, pj = p["permutation"][li], p["permutation"][lj]
         base = "inv_g_dt" if li == lj else "0.0"
         lst = jac_terms.get((pi, pj))
-        if (li, lj) in vm.lu_slot and lst is not None and (li, lj) in {tuple(x) for x in vm.w_locs}:
+        has_eq = pi in eq_row and (pj in eq_col[eq_row[pi][0]] or pi == pj)
+        if (li, lj) in vm.lu_slot and (lst is not None or has_eq) and (li, lj) in wset_log:
             jexpr = "0.0"
-            for gi, c in lst:
+            for gi, c in lst or []:
                 jexpr = f"({jexpr} + {_coef_times(c, f'd{gi}_{pj}')})"
+            jexpr = eq_j_terms(pi, pj, jexpr)
             w(f"    lu[{s}] = {base} - {jexpr};")
         else:
             # W location without a J entry (diagonal) or a fill position
@@ -382,10 +454,14 @@ def emit(mech: Mechanism, p: dict) -> str:
     refp = vm.yref_phys
     for gi in range(len(groups)):
         w(f"    const double fl{gi} = {_flux_expr(vm, groups[gi], f'k[{gi}]', ref=refp)};")
+    for st in eq_setup(False, refp):
+        w(st)
     for i in range(n):
         expr = "0.0"
         for gi, c in contrib[i]:
             expr = f"({expr} + {_coef_times(c, f'fl{gi}')})"
+        if i in eq_row:
+            expr = f"({expr} + {eq_f_term(i, refp)})"
         w(f"    f[{i}] = {expr};")
     w("  }")
     w("")
@@ -395,11 +471,14 @@ def emit(mech: Mechanism, p: dict) -> str:
     w("                                                  long stride) {")
     w("    (void)dk;")
     w(f"    for (int e = 0; e < {n * n}; ++e) J[e * stride] = 0.0;")
+    for st in eq_setup(True, refp):
+        w(st)
     for (i, j) in [tuple(e) for e in p["j_entries"]]:
         lst = jac_terms.get((i, j), [])
         jexpr = "0.0"
         for gi, c in lst:
             jexpr = f"({jexpr} + {_coef_times(c, _partial_expr(vm, groups[gi], j, f'k[{gi}]', f'dk[{gi}]', tb, ref=refp))})"
+        jexpr = eq_j_terms(i, j, jexpr)
         w(f"    J[{i + n * j}L * stride] = {jexpr};")
     w("  }")
     w("};")
@@ -413,9 +492,23 @@ LANE_MAX_VARIABLE_SPECIES = 32
 
 def emit_unit(p: dict) -> str:
     """Translation unit registering a lane-per-cell mechanism."""
+    from .emit_unrolled import MAX_SPECIES
+    from .model import NO_DIFF
+
     name = p["name"]
     sf = p["step_flops"]
     flops = ", ".join(f"{float(sf[m])!r}" for m in ("ros2", "ros3", "ros4", "rodas3", "rodas4"))
+    if name in NO_DIFF:
+        assert len(p["species"]) <= MAX_SPECIES
+        return (
+            f"// Generated by mkpp_amd.gen.emit_hip: registers {name} (one cell per lane;\n"
+            "// no differentiation kernels).\n"
+            f'#include "{name}.hpp"\n'
+            '#include "../runtime.hpp"\n'
+            '#include "../unrolled.hpp"\n'
+            f'#include "{name}_unrolled.hpp"\n\n'
+            f"MKPP_LANE_MECHANISM({name}, {flops}, nullptr)\n"
+        )
     return (
         f"// Generated by mkpp_amd.gen.emit_hip: registers {name} (one cell per lane;\n"
         "// differentiation on the wave engine's tables of the same mechanism).\n"

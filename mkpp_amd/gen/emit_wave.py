@@ -98,6 +98,8 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
     vm = VarMap(mech, p)
     nv, n = vm.nv, vm.n
     nfix = len(vm.fixed_phys)
+    if p.get("equilibria") or p.get("uses_cloud_water"):
+        raise ValueError("the wave engine has no equilibrium blocks or cloud-gated rates")
     if nv > 254:
         raise ValueError("wave engine supports up to 254 variable species (rows are bytes, 255 marks no row)")
     # ---- variable-logical LU pattern ---------------------------------------

@@ -205,11 +205,15 @@ class RateLaw:
         return (self.kind, self.photo_index, items)
 
     # -- numeric evaluation (Python float, for tests and fixtures) ------------
-    def value(self, T: float, m: float, jvals) -> float:
-        return self.value_and_dm(T, m, jvals)[0]
+    @property
+    def cloud_gated(self) -> bool:
+        return self.kind == "HETEROGENEOUS" and self.p["gate_width"].value != 0.0
 
-    def value_and_dm(self, T: float, m: float, jvals) -> tuple[float, float]:
-        """Return (k, dk/d[M])."""
+    def value(self, T: float, m: float, jvals, clw: float = 0.0) -> float:
+        return self.value_and_dm(T, m, jvals, clw)[0]
+
+    def value_and_dm(self, T: float, m: float, jvals, clw: float = 0.0) -> tuple[float, float]:
+        """Return (k, dk/d[M]); clw is the cell's cloud liquid water."""
         p = self.p
         if self.kind in ("PHOTOLYSIS", "PARAMETER"):
             return float(jvals[self.photo_index]), 0.0
@@ -221,7 +225,10 @@ class RateLaw:
         if self.kind == "CONSTANT":
             return p["A"].evaluate(T), 0.0
         if self.kind == "HETEROGENEOUS":
-            return 0.25 * p["gamma"].evaluate(T), 0.0
+            k = 0.25 * p["gamma"].evaluate(T)
+            if self.cloud_gated:
+                k *= cloud_gate(clw, p["gate_threshold"].value, p["gate_width"].value)
+            return k, 0.0
         if self.kind == "TROE":
             k0 = p["A0"].evaluate(T) * math.exp(p["C0"].evaluate(T) / T) * (T / 300.0) ** p["B0"].evaluate(T)
             ki = p["A1"].evaluate(T) * math.exp(p["C1"].evaluate(T) / T) * (T / 300.0) ** p["B1"].evaluate(T)
@@ -262,6 +269,25 @@ def _troe_pair(a, b, c):
     return A, B, C
 
 
+def _trigger_threshold(trigger) -> float:
+    """The number of ``meteo.cloud_liquid_water > <number>`` (lowering.py:141-160)."""
+    import re
+
+    m = re.search(r">\s*([-+0-9.eE]+)", str(trigger))
+    if not m:
+        raise MechanismError(f"activation_trigger without a '> <threshold>' comparison: {trigger!r}")
+    v = float(m.group(1))
+    if not math.isfinite(v):
+        raise MechanismError(f"activation_trigger threshold is not finite: {trigger!r}")
+    return v
+
+
+def cloud_gate(clw: float, threshold: float, width: float) -> float:
+    """1/2 (1 + u / sqrt(1 + u^2)) with u = (clw - threshold) / width."""
+    u = (clw - threshold) / width
+    return 0.5 * (1.0 + u / math.sqrt(1.0 + u * u))
+
+
 def _rate_law(rtype: str, r: dict, photo_index: int, reaction_index: int = -1) -> RateLaw:
     t = rtype.upper()
     if t == "PHOTOLYSIS":
@@ -287,9 +313,20 @@ def _rate_law(rtype: str, r: dict, photo_index: int, reaction_index: int = -1) -
     if t == "EP3":
         return RateLaw("EP3", {k: parse_param(r.get(k, 0.0)) for k in ("A1", "C1", "A2", "C2")})
     if t == "HETEROGENEOUS":
-        if "activation_trigger" in r:
-            raise MechanismError("cloud-gated HETEROGENEOUS reactions are outside this build")
-        return RateLaw("HETEROGENEOUS", {"gamma": parse_param(r["gamma"])})
+        # 0.25 gamma v_gas S_a with v_gas = S_a = 1, times a smooth cloud-water
+        # gate 1/2 (1 + u / sqrt(1 + u^2)), u = (CLW - threshold) / width, when
+        # the reaction has an activation trigger (lowering.py:310-313, 339-350;
+        # parser.py:284-305).  gate_width 0 marks an ungated reaction.
+        thr, width = 0.0, 0.0
+        trigger = r.get("activation_trigger")
+        if trigger is not None:
+            quantity = str(trigger).split(">")[0].strip()
+            if quantity != "meteo.cloud_liquid_water":
+                raise MechanismError(f"unsupported activation_trigger {trigger!r}")
+            thr = _trigger_threshold(trigger)
+            width = float(r.get("activation_width", max(abs(thr) * 1.0e-2, 1.0e-30)))
+        return RateLaw("HETEROGENEOUS", {"gamma": parse_param(r["gamma"]), "gate_threshold": parse_param(thr),
+                                         "gate_width": parse_param(width)})
     if "A" in r:
         return RateLaw("CONSTANT", {"A": parse_param(r["A"])})
     if t in ("USER_DEFINED", "SURFACE") and reaction_index >= 0:
@@ -354,11 +391,14 @@ class Mechanism:
         self.third_body = self.index.get("AIR", self.index.get("M", -1))
 
         self.reactions: list[Reaction] = []
+        self.equilibria: list[Equilibrium] = []
         photo = 0
         for ridx, r in enumerate(data.get("reactions", [])):
             rtype = r.get("type", "UNKNOWN")
             if rtype.upper() == "EQUILIBRIUM":
-                raise MechanismError("EQUILIBRIUM blocks are outside this build")
+                # not a kinetic reaction (parser.py:151-203)
+                self.equilibria.append(Equilibrium.parse(r, self))
+                continue
             if rtype.upper() == "SURFACE":
                 g = r.get("gas-phase species")
                 ins = {str(g): 1.0} if g else {}
@@ -458,12 +498,26 @@ class Mechanism:
             deps.append(self.third_body)
         return sorted(deps)
 
+    @property
+    def uses_cloud_water(self) -> bool:
+        return any(g.law.cloud_gated for g in self.groups)
+
     def _build_structure(self):
         nz = set()
         for g in self.groups:
             deps = self.group_depends(g)
             for i in g.nu:
                 for j in deps:
+                    nz.add((i, j))
+        # equilibrium relaxation tau (eq_i(totals) - C_i): every output row
+        # depends on every species of every element, and on itself
+        for e in self.equilibria:
+            cols = sorted({s for members in e.elements.values() for s in members})
+            for i in e.outputs:
+                if self.species[i].fixed:
+                    continue
+                nz.add((i, i))
+                for j in cols:
                     nz.add((i, j))
         # Row-major physical order, as jacobian_entries_state (template_context.py:494-509).
         self.jac_entries = [(i, j) for i in range(self.n) for j in range(self.n) if (i, j) in nz]
@@ -478,6 +532,8 @@ class Mechanism:
         return [1.0 if s.unit_value else float(y[i]) for i, s in enumerate(self.species)]
 
     def tendency(self, T: float, y, jvals, air_density: float = 2.4476e19):
+        if self.equilibria or self.uses_cloud_water:
+            raise MechanismError("the Python evaluation covers mass-action mechanisms only")
         yv = self._yv(y)
         ks = self.rate_constants(T, y, jvals, air_density)
         f = [0.0] * self.n
@@ -491,6 +547,8 @@ class Mechanism:
 
     def jacobian(self, T: float, y, jvals, air_density: float = 2.4476e19):
         """Dense J (list of rows) including fixed-species columns."""
+        if self.equilibria or self.uses_cloud_water:
+            raise MechanismError("the Python evaluation covers mass-action mechanisms only")
         yv = self._yv(y)
         ks = self.rate_constants(T, y, jvals, air_density)
         J = [[0.0] * self.n for _ in range(self.n)]
@@ -512,6 +570,59 @@ class Mechanism:
                 for i, c in g.nu.items():
                     J[i][j] += c * d
         return J
+
+
+# Partition models of EQUILIBRIUM blocks: system -> (element names the model
+# reads, species it returns, in the order of its outputs).  NH4_NO3_SO4 is the
+# reference's ISORROPIA-Lite-style model (src/mkpp/equilibrium/nh4_no3_so4.py);
+# sulfate enters only through the ratio C_N / C_S, SO2 and SO4 are not relaxed.
+PARTITION_MODELS = {
+    "NH4_NO3_SO4": (("reduced_nitrogen", "oxidized_nitrogen", "sulfate"),
+                    ("NH3", "NH4a", "HNO3", "NO3an1", "NO3an2", "NO3an3")),
+}
+
+
+@dataclass
+class Equilibrium:
+    """An EQUILIBRIUM block: F_i += tau_inv (eq_i(C_N, C_Ox, C_S, T) - C_i) for
+    the model's output species i (lowering.py:374-414), where each total is the
+    sum of its element's species ([gas] + aerosol, parser.py:160-168)."""
+
+    system: str
+    elements: dict  # element name -> [species index] (gas first)
+    outputs: list  # species index per model output, in PARTITION_MODELS order
+    width: float  # regime transition width in sulfate-ratio units
+    tau_inv: float  # relaxation rate, s^-1
+
+    @staticmethod
+    def parse(r: dict, mech: "Mechanism") -> "Equilibrium":
+        system = str(r.get("system", ""))
+        if system not in PARTITION_MODELS:
+            raise MechanismError(f"unsupported EQUILIBRIUM system {system!r}")
+        names, outs = PARTITION_MODELS[system]
+        elements = {}
+        for el, d in (r.get("total_species") or {}).items():
+            aer = d.get("aerosol", [])
+            aer = [aer] if isinstance(aer, str) else list(aer)
+            members = [d.get("gas", "")] + aer
+            for nm in members:
+                if nm not in mech.index:
+                    raise MechanismError(f"EQUILIBRIUM references unknown species {nm!r} in element {el!r}")
+            elements[el] = [mech.index[nm] for nm in members]
+        for el in names:
+            if el not in elements:
+                raise MechanismError(f"EQUILIBRIUM {system} needs the element {el!r}")
+        for nm in outs:
+            if nm not in mech.index:
+                raise MechanismError(f"EQUILIBRIUM {system} partitions {nm!r}, which the mechanism lacks")
+        own = [names[0], names[0], names[1], names[1], names[1], names[1]]
+        for nm, el in zip(outs, own):
+            if mech.index[nm] not in elements[el]:
+                raise MechanismError(f"EQUILIBRIUM {system}: {nm!r} must belong to the element {el!r}")
+        if str(r.get("regime_blending", "sigmoid")) != "sigmoid":
+            raise MechanismError("only sigmoid regime blending is supported")
+        return Equilibrium(system, elements, [mech.index[nm] for nm in outs],
+                           float(r.get("transition_width", 0.05)), float(r.get("relaxation_timescale_inv", 1e6)))
 
 
 def _species_dict(d) -> dict:
@@ -559,12 +670,15 @@ def load_mechanism(name_or_path: str) -> Mechanism:
 # decay (X -> nothing at a per-cell rate) is the scalar test equation of the
 # step-size controller's known-answer tests; decay_wave is the same mechanism
 # built on the wave-per-cell engine, so that those tests reach both engines.
-# ts1 (210 species) runs on the wave engine with four rows per lane.
-BUNDLED = ("chapman", "small_strato", "carbon", "saprc99", "saprcnov", "decay", "decay_wave", "ts1")
-# Wave mechanisms registered without the differentiation kernels (their
-# record, adjoint and tangent-linear instantiations would add minutes to every
-# build of a 49,000-product factorization).
-NO_DIFF = ("ts1",)
+# ts1 (210 species) runs on the wave engine with four rows per lane; gocart
+# (26 species, cloud-gated heterogeneous uptake and an NH4/NO3/SO4 equilibrium
+# block) on the lane engine.
+BUNDLED = ("chapman", "small_strato", "carbon", "saprc99", "saprcnov", "decay", "decay_wave", "ts1", "gocart")
+# Mechanisms registered without the differentiation kernels: ts1 (the record,
+# adjoint and tangent-linear instantiations would add minutes to every build
+# of a 49,000-product factorization) and gocart (the wave engine's tables that
+# the lane mechanisms differentiate on have no equilibrium block).
+NO_DIFF = ("ts1", "gocart")
 # Mechanisms built on the wave-per-cell engine whatever their size.
 FORCE_WAVE = ("decay_wave",)
 # Mechanisms of the persistent batch kernel (each tableau's kernel carries every

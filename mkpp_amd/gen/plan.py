@@ -23,7 +23,7 @@ LAW_PARAMS = {
     "TROE": ["A0", "B0", "C0", "A1", "B1", "C1", "Fc"],
     "EP2": ["A0", "C0", "A2", "C2", "A3", "C3"],
     "EP3": ["A1", "C1", "A2", "C2"],
-    "HETEROGENEOUS": ["gamma"],
+    "HETEROGENEOUS": ["gamma", "gate_threshold", "gate_width"],
     "CONSTANT": ["A"],
 }
 
@@ -72,6 +72,14 @@ def flop_counts(mech: Mechanism, plan) -> dict:
         deps = [j for j in mech.group_depends(g) if j not in fixed]
         for j in deps:
             j_flops += max(nfac - 1, 0) + 1 + 2 * len(g.nu)
+    # equilibrium blocks: the partition's operations (46 with the square roots,
+    # 8 for the element sums, 3 per output: 72 for F; the derivative pass adds
+    # 94, and each J entry one multiply-add, each output row its diagonal -tau)
+    for e in mech.equilibria:
+        rows = [i for i in e.outputs if i not in fixed]
+        cols = {s for m in e.elements.values() for s in m}
+        f_flops += 72
+        j_flops += 140 + 2 * len(rows) * len(cols) + len(rows)
     nv = mech.n - len(fixed)
     lu_products = 0
     for (i, j) in plan.lu_locations:
@@ -138,6 +146,9 @@ def build(mech: Mechanism) -> dict:
         "num_runtime_parameters": mech.num_runtime_parameters,
         "supports_runtime_environment": mech.has_runtime_environment,
         "num_reactions": len(mech.reactions),
+        "uses_cloud_water": mech.uses_cloud_water,
+        "equilibria": [{"system": e.system, "elements": e.elements, "outputs": e.outputs, "width": e.width,
+                        "tau_inv": e.tau_inv} for e in mech.equilibria],
         "groups": groups,
         "permutation": plan.permutation,
         "j_entries": [list(e) for e in plan.j_entries],
@@ -176,6 +187,17 @@ def write_oracle_table(p: dict, path: str) -> None:
         L.append("powers " + " ".join(f"{s} {q}" for s, q in g["powers"]))
         L.append("nu " + " ".join(f"{s} {c!r}" for s, c in g["nu"]))
         L.append("deps " + str(len(g["depends"])) + " " + " ".join(str(d) for d in g["depends"]))
+    # equilibrium blocks: system id (0 = NH4_NO3_SO4), tau_inv, width, then the
+    # species of each element the model reads and the output species
+    from .model import PARTITION_MODELS
+
+    L.append(f"equilibria {len(p['equilibria'])}")
+    for e in p["equilibria"]:
+        names, _ = PARTITION_MODELS[e["system"]]
+        L.append(f"eq {list(PARTITION_MODELS).index(e['system'])} {e['tau_inv']!r} {e['width']!r}")
+        for el in names:
+            L.append(f"element {len(e['elements'][el])} " + " ".join(str(x) for x in e["elements"][el]))
+        L.append(f"outputs {len(e['outputs'])} " + " ".join(str(x) for x in e["outputs"]))
     L.append("perm " + " ".join(str(x) for x in p["permutation"]))
     L.append(f"jslots {len(p['j_entries'])} " + " ".join(f"{i} {j}" for i, j in p["j_entries"]))
     L.append(f"wslots {len(p['w_locations'])} " + " ".join(f"{i} {j} {s}" for (i, j), s in zip(p["w_locations"], p["w_from_j"])))

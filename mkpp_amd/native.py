@@ -53,6 +53,7 @@ _SIGS = {
     "mkpp_amd_set_state_ptrs": ([_vp, _dp, _dp, _dp, _dp], C.c_int),
     "mkpp_amd_set_rate_parameter_ptrs": ([_vp, _dp, _sz], C.c_int),
     "mkpp_amd_set_photolysis_ptrs": ([_vp, _dp], C.c_int),
+    "mkpp_amd_set_cloud_water_ptr": ([_vp, _dp], C.c_int),
     "mkpp_amd_integrate": ([_vp, C.c_double], C.c_int),
     "mkpp_amd_integrate_steps": ([_vp, C.c_double, C.c_int], C.c_int),
     "mkpp_amd_evaluate_rates": ([_vp, _dp], C.c_int),
@@ -209,6 +210,13 @@ class Handle:
             assert rp.shape == (self.info["num_rate_parameters"], self.num_cells)
             self._keep.append(rp)
             self._check(lib().mkpp_amd_set_rate_parameter_ptrs(self._h, _d(rp), rp.shape[0]), "set_rate_parameter_ptrs")
+
+    def set_cloud_water(self, cloud_water):
+        """Per-cell cloud liquid water (the reference's mkpp_set_cloud_water_ptr);
+        cells read 0 until it is set."""
+        cw = np.ascontiguousarray(np.broadcast_to(cloud_water, (self.num_cells,)), dtype=np.float64)
+        self._keep_cw = cw
+        self._check(lib().mkpp_amd_set_cloud_water_ptr(self._h, _d(cw)), "set_cloud_water_ptr")
 
     def set_solver(self, method="ros3", atol=None, rtol=None, seed=1.0e-6):
         n = self.info["num_species"]

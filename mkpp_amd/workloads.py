@@ -16,6 +16,9 @@ tolerances the configuration integrates with.
 * saprcnov_urban  - saprcnov with the saprc99 urban setup and matched photolysis rates
 * ts1_scenario    - ts1 (mol m-3): the reference runner's binding data, Ros3 at the
                     measurement scenario's accuracy controls, 300 s + 300 s
+* gocart_mixed    - gocart: gases, aerosol bins, sulfate ratios across the
+                    NH4/NO3/SO4 partition's regimes and cloud water around the
+                    gate, Rodas3 adaptive, rtol 1e-3, atol 1e3, 1 h
 * batch_mixed     - the batching workload: saprc99 (NOx, VOCs 1e8-1e13, half night),
                     carbon and small_strato (half night) cells, Rodas3 adaptive
                     at rtol 1e-3 for all three in one launch
@@ -261,3 +264,46 @@ def ts1_scenario(num_cells: int = 1024, seed: int | None = 7, spread: float = 0.
             "dt": 300.0, "steps": 1, "seed": 1.0e-6, "atol": np.asarray(d["atol"]), "rtol": float(d["rtol"]),
             "photolysis_slots": photo,
             "segments": [(s["seconds"], s["photolysis_scale"]) for s in d["segments"]]}
+
+
+GOCART_CENTRES = {
+    # gases, molecules cm-3
+    "OH": 1e6, "NO3": 1e7, "H2O2": 1e10, "DMS": 1e9, "SO2": 1e10, "SO4": 1e10, "HNO3": 1e10, "NH3": 1e10,
+    "NH4a": 1e10, "NO3an1": 3e9, "NO3an2": 3e9, "NO3an3": 3e9,
+    # carbonaceous aerosol tracers
+    "BC1": 1e9, "BC2": 1e9, "OC1": 1e9, "OC2": 1e9,
+}
+
+
+def gocart_mixed(num_cells: int = 65536, seed: int = 9) -> dict:
+    """gocart cells: gases log-uniform within a decade of GOCART_CENTRES,
+    dust and sea-salt bins (the heterogeneous reactions' second reactant, read
+    as uptake surfaces with v_gas = S_a = 1) log-uniform in 1e-7..1e-5, the
+    sulfate ratio (NH3 + NH4a) / (SO2 + SO4) log-uniform in 0.2..20 (both
+    sides of the partition's regime boundaries at 1 and 2), T in 250..310 K,
+    H2O2 photolysis up to 1e-5 s-1 (the mechanism's host forcing), and cloud
+    liquid water 0 in a third of the cells and log-uniform in 1e-7..1e-5
+    (around the 1e-6 gate of SO2 + H2O2) elsewhere.  Rodas3 adaptive, rtol
+    1e-3, atol 1e3, 1 h."""
+    from .gen.model import load_mechanism
+
+    mech = load_mechanism("gocart")
+    names = mech.names
+    rng = np.random.default_rng(seed)
+    conc = np.zeros((len(names), num_cells))
+    for i, nm in enumerate(names):
+        if nm in GOCART_CENTRES:
+            conc[i] = GOCART_CENTRES[nm] * 10.0 ** rng.uniform(-1.0, 1.0, num_cells)
+        else:
+            conc[i] = 10.0 ** rng.uniform(-7.0, -5.0, num_cells)
+    cs = conc[names.index("SO2")] + conc[names.index("SO4")]
+    cn = cs * 10.0 ** rng.uniform(np.log10(0.2), np.log10(20.0), num_cells)
+    f = rng.uniform(0.1, 0.9, num_cells)
+    conc[names.index("NH3")], conc[names.index("NH4a")] = f * cn, (1.0 - f) * cn
+    T = rng.uniform(250.0, 310.0, num_cells)
+    p = rng.uniform(5e4, 1e5, num_cells)
+    clw = np.where(rng.uniform(0.0, 1.0, num_cells) < 1.0 / 3.0, 0.0, 10.0 ** rng.uniform(-7.0, -5.0, num_cells))
+    jv = rng.uniform(0.0, 1e-5, (1, num_cells))
+    return {"mechanism": "gocart", "conc": conc, "rate_parameters": jv, "temp": T, "pressure": p,
+            "air_density": number_density(p, T), "cloud_water": clw, "method": "rodas3", "dt": 3600.0,
+            "steps": 1, "seed": 1.0e-6, "atol": 1.0e3, "rtol": 1.0e-3}

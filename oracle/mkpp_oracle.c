@@ -8,8 +8,12 @@
  *
  *   rate laws ............ src/mkpp/lowering.py:199-313 (photolysis, Arrhenius,
  *                          Troe/falloff with the B/C swap and N = 1, EP2, EP3,
- *                          heterogeneous), third body AIR > M > air density
+ *                          heterogeneous with its cloud-water gate :339-350),
+ *                          third body AIR > M > air density
  *                          (lowering.py:283, 296, 307)
+ *   equilibrium blocks ... lowering.py:374-414 relaxation tau (eq_i - C_i) with
+ *                          the NH4/NO3/SO4 partition of
+ *                          src/mkpp/equilibrium/nh4_no3_so4.py:94-280
  *   F and J .............. lowering.py:315-372 mass-action scatter; J = dF/dC
  *                          with placeholder species read as 1.0 after
  *                          differentiation (src/mkpp/format_eqn.py:137-142)
@@ -62,10 +66,21 @@ typedef struct {
     int *dep;
 } group_t;
 
+/* EQUILIBRIUM block: element species (reduced N, oxidized N, sulfate) and the
+ * output species (NH3, NH4a, HNO3, NO3an1, NO3an2, NO3an3). */
+typedef struct {
+    int system;
+    double tau_inv, width;
+    int nel[3], *el[3];
+    int nout, out[6];
+} eq_t;
+
 typedef struct {
     int n, nphoto, third_body, ngroups;
     int *fixed, *unit;
     group_t *g;
+    int neq;
+    eq_t *eq;
     int *perm;
     int nj, *ji, *jj, *jslot_of; /* jslot_of[i*n+j] physical -> slot or -1 */
     int nw, *wi, *wj, *wfromj;
@@ -175,6 +190,22 @@ mech_t *oracle_load(const char *path) {
         g->dep = ialloc(g->ndep);
         for (int k = 0; k < g->ndep; ++k) ok &= fscanf(f, "%d", &g->dep[k]) == 1;
     }
+    ok &= !expect(f, "equilibria") && fscanf(f, "%d", &m->neq) == 1;
+    if (!ok) goto fail;
+    m->eq = (eq_t *)calloc(m->neq > 0 ? m->neq : 1, sizeof(eq_t));
+    for (int b = 0; b < m->neq && ok; ++b) {
+        eq_t *e = &m->eq[b];
+        ok &= !expect(f, "eq") && fscanf(f, "%d %lf %lf", &e->system, &e->tau_inv, &e->width) == 3;
+        for (int el = 0; el < 3 && ok; ++el) {
+            ok &= !expect(f, "element") && fscanf(f, "%d", &e->nel[el]) == 1;
+            if (!ok) break;
+            e->el[el] = ialloc(e->nel[el]);
+            for (int k = 0; k < e->nel[el]; ++k) ok &= fscanf(f, "%d", &e->el[el][k]) == 1;
+        }
+        ok &= !expect(f, "outputs") && fscanf(f, "%d", &e->nout) == 1;
+        if (!ok || e->nout != 6 || e->system != 0) goto fail;
+        for (int k = 0; k < e->nout; ++k) ok &= fscanf(f, "%d", &e->out[k]) == 1;
+    }
     m->perm = ialloc(m->n);
     ok &= !expect(f, "perm");
     for (int i = 0; i < m->n; ++i) ok &= fscanf(f, "%d", &m->perm[i]) == 1;
@@ -246,6 +277,9 @@ void oracle_free(mech_t *m) {
         free(m->g[q].dep);
     }
     free(m->g);
+    for (int b = 0; b < m->neq && m->eq; ++b)
+        for (int el = 0; el < 3; ++el) free(m->eq[b].el[el]);
+    free(m->eq);
     free(m->fixed); free(m->unit); free(m->perm);
     free(m->ji); free(m->jj); free(m->jslot_of);
     free(m->wi); free(m->wj); free(m->wfromj);
@@ -271,8 +305,9 @@ static double third_body_value(const mech_t *m, const double *y, double air_dens
     return m->third_body >= 0 ? y[m->third_body] : air_density;
 }
 
-/* k and dk/d[M] of one flux group. */
-static void group_rate(const mech_t *m, const group_t *g, double T, double M, const double *jv, double *k, double *dk) {
+/* k and dk/d[M] of one flux group; clw is the cell's cloud liquid water. */
+static void group_rate(const mech_t *m, const group_t *g, double T, double M, const double *jv, double clw, double *k,
+                       double *dk) {
     const rpn_t *p = g->param;
     *dk = 0.0;
     switch (g->law) {
@@ -314,13 +349,25 @@ static void group_rate(const mech_t *m, const group_t *g, double T, double M, co
         *dk = K2;
         break;
     }
-    case LAW_HET: *k = 0.25 * eval_rpn(&p[0], T); break;
+    case LAW_HET: {
+        /* 0.25 gamma v_gas S_a (v_gas = S_a = 1), gated by 1/2 (1 + u / sqrt(1 + u^2)),
+         * u = (CLW - threshold) / width, where the width is not 0 */
+        *k = 0.25 * eval_rpn(&p[0], T);
+        double width = eval_rpn(&p[2], T);
+        if (width != 0.0) {
+            double u = (clw - eval_rpn(&p[1], T)) / width, q = sqrt(1.0 + u * u);
+            *k *= u >= 0.0 ? 0.5 * (1.0 + u / q) : 0.5 / (q * (q - u)); /* the same, without cancelling */
+        }
+        break;
+    }
     default: *k = eval_rpn(&p[0], T); break;
     }
 }
 
+static inline double yval(const mech_t *m, const double *y, int s) { return m->unit[s] ? 1.0 : y[s]; }
+
 typedef struct {
-    double T, air_density;
+    double T, air_density, clw;
     const double *jv;
     int hoist;
     double *k, *dk; /* per group, valid when hoist */
@@ -328,10 +375,114 @@ typedef struct {
 
 static void env_rates(const mech_t *m, env_t *e, const double *y) {
     double M = third_body_value(m, y, e->air_density);
-    for (int q = 0; q < m->ngroups; ++q) group_rate(m, &m->g[q], e->T, M, e->jv, &e->k[q], &e->dk[q]);
+    for (int q = 0; q < m->ngroups; ++q) group_rate(m, &m->g[q], e->T, M, e->jv, e->clw, &e->k[q], &e->dk[q]);
 }
 
-static inline double yval(const mech_t *m, const double *y, int s) { return m->unit[s] ? 1.0 : y[s]; }
+/* ------------------------------------------------------------------ */
+/* NH4/NO3/SO4 partition (src/mkpp/equilibrium/nh4_no3_so4.py:94-280)  */
+/* ------------------------------------------------------------------ */
+/* v[6]: equilibrium NH3, NH4a, HNO3, NO3an1, NO3an2, NO3an3 from the totals
+ * CN (NH3 + NH4a), COx (HNO3 + NO3an*), CS (SO2 + SO4) at temperature T.
+ * With d (d[6][3], may be NULL): their derivatives in CN, COx, CS, and
+ * dd[6] = d[k][own element] - 1, the diagonal of the relaxation's Jacobian
+ * over tau_inv (NH3 and NH4a belong to CN, the others to COx), formed without
+ * the cancellation of 1 - 1.
+ *   R = CN / (CS + 1e-30), w_i = 1/2 (1 + u_i / sqrt(1 + u_i^2)),
+ *   u_1 = (R - 1) / width, u_2 = (R - 2) / width,
+ *   Kp = 4.39e-17 exp(-dH/Rgas (1/T - 1/298.15)), dH = -74735 J/mol, Rgas = 8.314,
+ *   e = CN - 2 CS, X = 1/2 ((e + COx) - sqrt((e - COx)^2 + 4 Kp)),
+ *   sp(x) = 1/2 (x + sqrt(x^2 + 1e-20)), X+ = sp(X),
+ *   NH3 = w_2 sp(e - X+), NH4a = CN - NH3, A = w_1 w_2 X+,
+ *   NO3an_k = A / 3, HNO3 = COx - A.
+ * Terms that cancel are taken in their equivalent closed forms: for x < 0,
+ * sp(x) = 1e-20 / (2 (s - x)) and sp'(x) = 1e-20 / (2 s (s - x)),
+ * s = sqrt(x^2 + 1e-20); 1 -+ g / sqrt(g^2 + 4 Kp) = 4 Kp / (sqrt(.) (sqrt(.)
+ * +- g)) on the side where g = e - COx has the cancelling sign; X =
+ * 2 (e COx - Kp) / ((e + COx) + sqrt(.)) where e + COx > 0; e - X+ through
+ * e - X = (g + sqrt(.)) / 2.  The reference's double-precision code evaluates
+ * the first forms; its exact values, which the golden vectors hold, are the
+ * same. */
+static void partition_nh4no3so4(double CN, double COx, double CS, double T, double width, double *v, double (*d)[3],
+                                double *dd) {
+    const double Kp = 4.39e-17 * exp((74735.0 / 8.314) * (1.0 / T - 1.0 / 298.15));
+    const double den = CS + 1e-30;
+    const double R = CN / den;
+    const double iw = 1.0 / width;
+    const double u1 = (R - 1.0) * iw, u2 = (R - 2.0) * iw;
+    const double q1 = sqrt(1.0 + u1 * u1), q2 = sqrt(1.0 + u2 * u2);
+    // w = (1 + u / q) / 2 and its complement 1 - w = (1 - u / q) / 2, each in the
+    // form that does not cancel: 1 / (2 q (q -+ u))
+    const double w1 = u1 >= 0.0 ? 0.5 * (1.0 + u1 / q1) : 0.5 / (q1 * (q1 - u1));
+    const double w2 = u2 >= 0.0 ? 0.5 * (1.0 + u2 / q2) : 0.5 / (q2 * (q2 - u2));
+    const double cw2 = u2 >= 0.0 ? 0.5 / (q2 * (q2 + u2)) : 0.5 * (1.0 - u2 / q2);
+    const double e = CN - 2.0 * CS;
+    const double g = e - COx;
+    const double sD = sqrt(g * g + 4.0 * Kp);
+    // X = ((e + COx) - sD) / 2 = 2 (e COx - Kp) / ((e + COx) + sD)
+    const double X = (e + COx) > 0.0 ? 2.0 * (e * COx - Kp) / ((e + COx) + sD) : 0.5 * ((e + COx) - sD);
+    const double sX = sqrt(X * X + 1e-20);
+    const double Xp = X >= 0.0 ? 0.5 * (X + sX) : 0.5 * 1e-20 / (sX - X);
+    // H = e - X+ = (e - X) - (X+ - X), e - X = (g + sD) / 2 = 2 Kp / (sD - g) for g < 0
+    const double eX = g >= 0.0 ? 0.5 * (g + sD) : 2.0 * Kp / (sD - g);
+    const double H = eX - (X >= 0.0 ? 0.5 * 1e-20 / (sX + X) : Xp - X);
+    const double sH = sqrt(H * H + 1e-20);
+    const double NH3h = H >= 0.0 ? 0.5 * (H + sH) : 0.5 * 1e-20 / (sH - H);
+    const double NH3 = w2 * NH3h;
+    const double A = w1 * w2 * Xp;
+    v[0] = NH3;
+    v[1] = CN - NH3;
+    v[2] = COx - A;
+    v[3] = v[4] = v[5] = A * (1.0 / 3.0);
+    if (!d) return;
+    /* 1 - g/sD and 1 + g/sD */
+    double am, ap;
+    if (g >= 0.0) {
+        am = 4.0 * Kp / (sD * (sD + g));
+        ap = 1.0 + g / sD;
+    } else {
+        am = 1.0 - g / sD;
+        ap = 4.0 * Kp / (sD * (sD - g));
+    }
+    const double dR[3] = {1.0 / den, 0.0, -R / den};
+    const double dX[3] = {0.5 * am, 0.5 * ap, -am};
+    const double s1 = 0.5 / (q1 * q1 * q1) * iw, s2 = 0.5 / (q2 * q2 * q2) * iw;
+    const double spX = X >= 0.0 ? 0.5 * (1.0 + X / sX) : 0.5 * 1e-20 / (sX * (sX - X));
+    const double spH = H >= 0.0 ? 0.5 * (1.0 + H / sH) : 0.5 * 1e-20 / (sH * (sH - H));
+    const double cH = H >= 0.0 ? 0.5 * 1e-20 / (sH * (sH + H)) : 0.5 * (1.0 - H / sH);  // 1 - sp'(H)
+    // de - dX+ for CN (for CS: -2 times it): 1 - sp'(X) am / 2 = cX + ap/2 - cX ap/2,
+    // cX = 1 - sp'(X) (am + ap = 2), which cancels nothing
+    const double cX = X >= 0.0 ? 0.5 * 1e-20 / (sX * (sX + X)) : 0.5 * (1.0 - X / sX);
+    const double t0 = cX + 0.5 * ap - cX * (0.5 * ap);
+    const double dedXp[3] = {t0, -spX * (0.5 * ap), -2.0 * t0};
+    // 1 - dNH3/dCN = (1 - w2) + w2 (1 - sp'(H)) + w2 sp'(H) (1 - t0) - dw2/dCN NH3h
+    const double one_t0 = (1.0 - cX) * (1.0 - 0.5 * ap);
+    const double cdN = cw2 + w2 * cH + w2 * spH * one_t0 - (s2 / den) * NH3h;
+    for (int c = 0; c < 3; ++c) {
+        double dw1 = s1 * dR[c], dw2 = s2 * dR[c];
+        double dXp = spX * dX[c];
+        double dNH3 = dw2 * NH3h + w2 * (spH * dedXp[c]);
+        double dA = (dw1 * w2 + w1 * dw2) * Xp + w1 * w2 * dXp;
+        d[0][c] = dNH3;
+        d[1][c] = (c == 0 ? 1.0 : 0.0) - dNH3;
+        d[2][c] = (c == 1 ? 1.0 : 0.0) - dA;
+        d[3][c] = d[4][c] = d[5][c] = dA * (1.0 / 3.0);
+        if (c == 0) {
+            d[1][0] = cdN;
+            dd[0] = -cdN;
+            dd[1] = -dNH3;
+        } else if (c == 1) {
+            dd[2] = -dA;
+            dd[3] = dd[4] = dd[5] = dA * (1.0 / 3.0) - 1.0;
+        }
+    }
+}
+
+static void eq_totals(const mech_t *m, const eq_t *e, const double *y, double *tot) {
+    for (int el = 0; el < 3; ++el) {
+        tot[el] = 0.0;
+        for (int k = 0; k < e->nel[el]; ++k) tot[el] += yval(m, y, e->el[el][k]);
+    }
+}
 
 static inline double ipow(double x, int p) {
     double r = x;
@@ -350,8 +501,18 @@ static void eval_rates(const mech_t *m, env_t *e, const double *y, double *f) {
         for (int t = 0; t < g->nnu; ++t) {
             double k, dk;
             if (e->hoist) k = e->k[q];
-            else group_rate(m, g, e->T, M, e->jv, &k, &dk);
+            else group_rate(m, g, e->T, M, e->jv, e->clw, &k, &dk);
             f[g->nu_s[t]] += g->nu_c[t] * (k * mono);
+        }
+    }
+    for (int b = 0; b < m->neq; ++b) {
+        const eq_t *q = &m->eq[b];
+        double tot[3], v[6];
+        eq_totals(m, q, y, tot);
+        partition_nh4no3so4(tot[0], tot[1], tot[2], e->T, q->width, v, NULL, NULL);
+        for (int k = 0; k < q->nout; ++k) {
+            int i = q->out[k];
+            if (!m->fixed[i]) f[i] += q->tau_inv * (v[k] - y[i]);
         }
     }
 }
@@ -367,7 +528,7 @@ static void eval_jacobian(const mech_t *m, env_t *e, const double *y, double *jv
             for (int t = 0; t < g->nnu; ++t) {
                 double k, dk;
                 if (e->hoist) { k = e->k[q]; dk = e->dk[q]; }
-                else group_rate(m, g, e->T, M, e->jv, &k, &dk);
+                else group_rate(m, g, e->T, M, e->jv, e->clw, &k, &dk);
                 double part = 0.0;
                 for (int a = 0; a < g->npow; ++a) {
                     if (g->pow_s[a] != j) continue;
@@ -384,6 +545,21 @@ static void eval_jacobian(const mech_t *m, env_t *e, const double *y, double *jv
                 int slot = m->jslot_of[g->nu_s[t] * m->n + j];
                 jv[slot] += g->nu_c[t] * part;
             }
+        }
+    }
+    for (int b = 0; b < m->neq; ++b) {
+        const eq_t *q = &m->eq[b];
+        double tot[3], v[6], d[6][3], dd[6];
+        eq_totals(m, q, y, tot);
+        partition_nh4no3so4(tot[0], tot[1], tot[2], e->T, q->width, v, d, dd);
+        for (int k = 0; k < q->nout; ++k) {
+            int i = q->out[k];
+            if (m->fixed[i]) continue;
+            for (int el = 0; el < 3; ++el)
+                for (int a = 0; a < q->nel[el]; ++a) {
+                    int j = q->el[el][a];
+                    jv[m->jslot_of[i * m->n + j]] += q->tau_inv * (j == i ? dd[k] : d[k][el]);
+                }
         }
     }
 }
@@ -732,20 +908,23 @@ static long integrate_cell_reduced(const mech_t *m, const tab_t *tb, double *sta
 /* ------------------------------------------------------------------ */
 
 /* F at one state, physical order. */
-void oracle_rates(const mech_t *m, double T, double air_density, const double *y, const double *jv, double *f) {
-    env_t e = {T, air_density, jv, 0, NULL, NULL};
+void oracle_rates(const mech_t *m, double T, double air_density, const double *y, const double *jv, double *f,
+                  double clw) {
+    env_t e = {T, air_density, clw, jv, 0, NULL, NULL};
     eval_rates(m, &e, y, f);
 }
 
 /* J values per J slot. */
-void oracle_jacobian(const mech_t *m, double T, double air_density, const double *y, const double *jv, double *jslots) {
-    env_t e = {T, air_density, jv, 0, NULL, NULL};
+void oracle_jacobian(const mech_t *m, double T, double air_density, const double *y, const double *jv, double *jslots,
+                     double clw) {
+    env_t e = {T, air_density, clw, jv, 0, NULL, NULL};
     eval_jacobian(m, &e, y, jslots);
 }
 
 /* Rate constant and dk/d[M] per flux group. */
-void oracle_rate_constants(const mech_t *m, double T, double air_density, const double *y, const double *jv, double *k, double *dk) {
-    env_t e = {T, air_density, jv, 1, k, dk};
+void oracle_rate_constants(const mech_t *m, double T, double air_density, const double *y, const double *jv, double *k,
+                           double *dk, double clw) {
+    env_t e = {T, air_density, clw, jv, 1, k, dk};
     env_rates(m, &e, y);
 }
 
@@ -753,12 +932,14 @@ void oracle_rate_constants(const mech_t *m, double T, double air_density, const 
  * Integrate num_cells cells, each `steps` times over dt_total, in the
  * C-ABI's LayoutLeft layout: conc[cell + num_cells*species],
  * jvals[cell + num_cells*slot]; temp and air_density per cell.
- * accepted/attempted (may be NULL) receive per-cell totals.
+ * accepted/attempted (may be NULL) receive per-cell totals; cloud_water (may be
+ * NULL: 0) the cells' cloud liquid water.
  * Returns the number of cells that hit max_attempts (0 = none).
  */
 int oracle_integrate(const mech_t *m, int method, long num_cells, double *conc, const double *temp, const double *air_density,
                      const double *jvals, double dt_total, int steps, double seed, const double *atol, const double *rtol,
-                     long *accepted, long *attempted, int hoist, long max_attempts, int nthreads) {
+                     long *accepted, long *attempted, int hoist, long max_attempts, int nthreads,
+                     const double *cloud_water) {
     tab_t tb;
     get_tableau(method, &tb);
     int stalled = 0;
@@ -777,7 +958,8 @@ int oracle_integrate(const mech_t *m, int method, long num_cells, double *conc, 
         for (long c = 0; c < num_cells; ++c) {
             for (int i = 0; i < m->n; ++i) state[i] = conc[c + num_cells * i];
             for (int p = 0; p < m->nphoto; ++p) jv[p] = jvals ? jvals[c + num_cells * p] : 0.0;
-            env_t e = {temp ? temp[c] : 300.0, air_density ? air_density[c] : 2.4476e19, jv, hoist, s.k, s.dk};
+            env_t e = {temp ? temp[c] : 300.0, air_density ? air_density[c] : 2.4476e19,
+                       cloud_water ? cloud_water[c] : 0.0, jv, hoist, s.k, s.dk};
             long acc_total = 0, att_total = 0;
             for (int st = 0; st < steps; ++st) {
                 long acc = 0;
@@ -803,12 +985,12 @@ int oracle_integrate(const mech_t *m, int method, long num_cells, double *conc, 
  * Returns the number of attempted steps; *accepted the accepted ones. */
 long oracle_integrate_trace(const mech_t *m, int method, double *state, double temp, double air_density,
                             const double *jv, double dt_total, double seed, const double *atol, const double *rtol,
-                            long max_attempts, double *trace, long trace_cap, long *accepted) {
+                            long max_attempts, double *trace, long trace_cap, long *accepted, double clw) {
     tab_t tb;
     get_tableau(method, &tb);
     scratch_t s;
     scratch_alloc(m, &s);
-    env_t e = {temp, air_density, jv, 1, s.k, s.dk};
+    env_t e = {temp, air_density, clw, jv, 1, s.k, s.dk};
     long acc = 0;
     long att = integrate_cell(m, &tb, state, &e, dt_total, seed, atol, rtol, &s, &acc, max_attempts, trace, trace_cap);
     scratch_free(&s);
@@ -838,7 +1020,8 @@ int oracle_tableau(int method, double *out) {
 int oracle_integrate_reduced(const mech_t *m, int method, long num_cells, double *conc, const double *temp,
                              const double *air_density, const double *jvals, double dt_total, int steps,
                              double threshold, const double *atol, const double *rtol, long *accepted,
-                             long *attempted, int hoist, long max_attempts, int *active_out, int nthreads) {
+                             long *attempted, int hoist, long max_attempts, int *active_out, int nthreads,
+                             const double *cloud_water) {
     tab_t tb;
     get_tableau(method, &tb);
     int stalled = 0;
@@ -858,7 +1041,8 @@ int oracle_integrate_reduced(const mech_t *m, int method, long num_cells, double
         for (long c = 0; c < num_cells; ++c) {
             for (int i = 0; i < m->n; ++i) state[i] = conc[c + num_cells * i];
             for (int p = 0; p < m->nphoto; ++p) jv[p] = jvals ? jvals[c + num_cells * p] : 0.0;
-            env_t e = {temp ? temp[c] : 300.0, air_density ? air_density[c] : 2.4476e19, jv, hoist, s.k, s.dk};
+            env_t e = {temp ? temp[c] : 300.0, air_density ? air_density[c] : 2.4476e19,
+                       cloud_water ? cloud_water[c] : 0.0, jv, hoist, s.k, s.dk};
             long acc_total = 0, att_total = 0;
             for (int st = 0; st < steps; ++st) {
                 long acc = 0;

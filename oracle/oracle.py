@@ -37,29 +37,33 @@ def lib():
         for nm in ("oracle_n", "oracle_nphoto", "oracle_nj", "oracle_nw", "oracle_nlu", "oracle_ngroups"):
             getattr(L, nm).argtypes = [vp]
             getattr(L, nm).restype = C.c_int
-        L.oracle_rates.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp]
-        L.oracle_jacobian.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp]
-        L.oracle_rate_constants.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp, dp]
+        L.oracle_rates.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp, C.c_double]
+        L.oracle_jacobian.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp, C.c_double]
+        L.oracle_rate_constants.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp, dp, C.c_double]
         L.oracle_factorize.argtypes = [vp, dp, dp]
         L.oracle_solve.argtypes = [vp, dp, dp, dp, dp]
         L.oracle_integrate.restype = C.c_int
         L.oracle_integrate.argtypes = [vp, C.c_int, C.c_long, dp, dp, dp, dp, C.c_double, C.c_int, C.c_double,
-                                       dp, dp, lp, lp, C.c_int, C.c_long, C.c_int]
+                                       dp, dp, lp, lp, C.c_int, C.c_long, C.c_int, dp]
         L.oracle_integrate_trace.restype = C.c_long
         L.oracle_integrate_trace.argtypes = [vp, C.c_int, dp, C.c_double, C.c_double, dp, C.c_double, C.c_double,
-                                             dp, dp, C.c_long, dp, C.c_long, lp]
+                                             dp, dp, C.c_long, dp, C.c_long, lp, C.c_double]
         L.oracle_tableau.restype = C.c_int
         L.oracle_tableau.argtypes = [C.c_int, dp]
         L.oracle_integrate_reduced.restype = C.c_int
         L.oracle_integrate_reduced.argtypes = [vp, C.c_int, C.c_long, dp, dp, dp, dp, C.c_double, C.c_int,
                                                C.c_double, dp, dp, lp, lp, C.c_int, C.c_long, C.POINTER(C.c_int),
-                                               C.c_int]
+                                               C.c_int, dp]
         _lib = L
     return _lib
 
 
 def _d(a):
     return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _cells(v, ncell):
+    return None if v is None else np.ascontiguousarray(np.broadcast_to(v, (ncell,)), dtype=np.float64)
 
 
 def _l(a):
@@ -90,26 +94,26 @@ class Oracle:
         except Exception:
             pass
 
-    def rates(self, y, jvals=(), T=300.0, air_density=2.4476e19):
+    def rates(self, y, jvals=(), T=300.0, air_density=2.4476e19, clw=0.0):
         y = np.ascontiguousarray(y, dtype=np.float64)
         jv = np.ascontiguousarray(np.asarray(jvals, dtype=np.float64).reshape(-1) if len(jvals) else np.zeros(1))
         f = np.zeros(self.n)
-        lib().oracle_rates(self._h, T, air_density, _d(y), _d(jv), _d(f))
+        lib().oracle_rates(self._h, T, air_density, _d(y), _d(jv), _d(f), float(clw))
         return f
 
-    def jacobian_slots(self, y, jvals=(), T=300.0, air_density=2.4476e19):
+    def jacobian_slots(self, y, jvals=(), T=300.0, air_density=2.4476e19, clw=0.0):
         y = np.ascontiguousarray(y, dtype=np.float64)
         jv = np.ascontiguousarray(np.asarray(jvals, dtype=np.float64).reshape(-1) if len(jvals) else np.zeros(1))
         out = np.zeros(self.nj)
-        lib().oracle_jacobian(self._h, T, air_density, _d(y), _d(jv), _d(out))
+        lib().oracle_jacobian(self._h, T, air_density, _d(y), _d(jv), _d(out), float(clw))
         return out
 
-    def rate_constants(self, y, jvals=(), T=300.0, air_density=2.4476e19):
+    def rate_constants(self, y, jvals=(), T=300.0, air_density=2.4476e19, clw=0.0):
         y = np.ascontiguousarray(y, dtype=np.float64)
         jv = np.ascontiguousarray(np.asarray(jvals, dtype=np.float64).reshape(-1) if len(jvals) else np.zeros(1))
         k = np.zeros(max(self.ngroups, 1))
         dk = np.zeros(max(self.ngroups, 1))
-        lib().oracle_rate_constants(self._h, T, air_density, _d(y), _d(jv), _d(k), _d(dk))
+        lib().oracle_rate_constants(self._h, T, air_density, _d(y), _d(jv), _d(k), _d(dk), float(clw))
         return k[: self.ngroups], dk[: self.ngroups]
 
     def factorize(self, w):
@@ -127,9 +131,10 @@ class Oracle:
         return x
 
     def integrate(self, conc, dt, steps=1, method="rodas3", temp=None, air_density=None, jvals=None,
-                  atol=None, rtol=None, seed=1.0e-6, hoist=True, max_attempts=0, threads=0):
+                  atol=None, rtol=None, seed=1.0e-6, hoist=True, max_attempts=0, threads=0, cloud_water=None):
         """Integrate cells in place.  conc: (num_species, num_cells) float64,
-        i.e. the LayoutLeft buffer conc[cell + num_cells*species].
+        i.e. the LayoutLeft buffer conc[cell + num_cells*species]; cloud_water:
+        per-cell cloud liquid water (None: 0).
         Returns (accepted, attempted, stalled_cells)."""
         conc = np.asarray(conc)
         assert conc.dtype == np.float64 and conc.flags.c_contiguous and conc.shape[0] == self.n
@@ -146,11 +151,11 @@ class Oracle:
         att = np.zeros(ncell, dtype=np.int64)
         stalled = lib().oracle_integrate(self._h, METHODS[method], ncell, _d(conc), _d(T), _d(rho), _d(jv), float(dt),
                                          int(steps), float(seed), _d(at), _d(rt), _l(acc), _l(att), int(bool(hoist)),
-                                         int(max_attempts), int(threads))
+                                         int(max_attempts), int(threads), _d(_cells(cloud_water, ncell)))
         return acc, att, stalled
 
     def integrate_trace(self, y, dt, method="rodas3", temp=300.0, air_density=2.4476e19, jvals=(), atol=1.0e-3,
-                        rtol=1.0e-6, seed=1.0e-6, max_attempts=0, cap=100000):
+                        rtol=1.0e-6, seed=1.0e-6, max_attempts=0, cap=100000, clw=0.0):
         """One cell's integrate(dt) with its attempt trace.  y: physical-order
         state (not modified).  Returns (y_end, trace) with trace an
         (attempts, 4) array of [t, dt, error norm, accepted] per attempt."""
@@ -163,12 +168,13 @@ class Oracle:
         acc = np.zeros(1, dtype=np.int64)
         att = lib().oracle_integrate_trace(self._h, METHODS[method], _d(y), float(temp), float(air_density), _d(jv),
                                            float(dt), float(seed), _d(at), _d(rt), int(max_attempts), _d(tr), int(cap),
-                                           _l(acc))
+                                           _l(acc), float(clw))
         assert att <= cap, "trace capacity exceeded"
         return y, tr[:att].copy()
 
     def integrate_reduced(self, conc, dt, steps=1, method="rodas3", temp=None, air_density=None, jvals=None,
-                          atol=None, rtol=None, threshold=0.0, hoist=True, max_attempts=0, threads=0):
+                          atol=None, rtol=None, threshold=0.0, hoist=True, max_attempts=0, threads=0,
+                          cloud_water=None):
         """integrate_with_reduction over cells in place (layout as integrate).
         atol/rtol are indexed by the solver's permuted position, as the
         reference's reduction variant indexes them.  Returns (accepted,
@@ -191,7 +197,8 @@ class Oracle:
         stalled = lib().oracle_integrate_reduced(self._h, METHODS[method], ncell, _d(conc), _d(T), _d(rho), _d(jv),
                                                  float(dt), int(steps), float(threshold), _d(at), _d(rt), _l(acc),
                                                  _l(att), int(bool(hoist)), int(max_attempts),
-                                                 active.ctypes.data_as(C.POINTER(C.c_int)), int(threads))
+                                                 active.ctypes.data_as(C.POINTER(C.c_int)), int(threads),
+                                                 _d(_cells(cloud_water, ncell)))
         return acc, att, stalled, active.astype(bool)
 
 

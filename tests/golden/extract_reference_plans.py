@@ -27,7 +27,7 @@ import sys
 
 REF = "/root/reference/mkpp-generated"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "plans")
-MECHS = ["chapman", "small_strato", "carbon", "saprc99", "saprc99_mini", "saprcnov", "ts1"]
+MECHS = ["chapman", "small_strato", "carbon", "saprc99", "saprc99_mini", "saprcnov", "ts1", "gocart"]
 
 
 def _ints(text):

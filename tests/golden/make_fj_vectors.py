@@ -29,7 +29,26 @@ import numpy as np
 REF_SRC = "/root/reference/src"
 REF_MECH = "/root/reference/mechanisms/openatmos"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fj")
-MECHS = ["chapman", "small_strato", "carbon", "saprc99_mini", "saprc99", "saprcnov", "ts1"]
+MECHS = ["chapman", "small_strato", "carbon", "saprc99_mini", "saprc99", "saprcnov", "ts1", "gocart"]
+
+
+def gocart_cases(rng, species):
+    """States across the NH4/NO3/SO4 partition's regimes (sulfate ratio
+    R = (NH3 + NH4a) / (SO2 + SO4) below 1, between 1 and 2, near 2 and above
+    it; totals where 4 Kp matters beside (e - COx)^2) and cloud water below,
+    at and above the 1e-6 gate of SO2 + H2O2 -> SO4."""
+    out = []
+    designs = [(0.5, 0.0, 1e-2, 290.0), (1.5, 2.0e-6, 1e-2, 270.0), (2.6, 1.0e-6, 1e-2, 300.0),
+               (1.99, 1.0e-6 + 5.0e-9, 1e-2, 285.0), (10.0, 0.0, 1e-2, 250.0), (3.0, 3.0e-7, 1e-8, 298.15)]
+    for ratio, clw, scale, T in designs:
+        y = scale * 10.0 ** rng.uniform(-2.0, 0.0, len(species))
+        ix = {s: i for i, s in enumerate(species)}
+        cs = y[ix["SO2"]] + y[ix["SO4"]]
+        cn = ratio * cs
+        f = rng.uniform(0.2, 0.8)
+        y[ix["NH3"]], y[ix["NH4a"]] = f * cn, (1.0 - f) * cn
+        out.append((y, T, clw))
+    return out
 
 
 def main():
@@ -61,7 +80,13 @@ def main():
         def prep(e):
             e = sp.sympify(str(e))  # re-parse as the printer does (format_eqn.py:133)
             e = e.subs(subs_base)
-            return e.subs({sp.Symbol("TEMP"): sp.Symbol("Temp"), sp.Symbol("temp"): sp.Symbol("Temp")})
+            e = e.subs({sp.Symbol("TEMP"): sp.Symbol("Temp"), sp.Symbol("temp"): sp.Symbol("Temp")})
+            if name == "gocart":
+                # carry the constants' values at 40 digits, so that nodes such as
+                # 1/2 + u / (2 sqrt(u^2 + 1)) at u = -100 are not evaluated at the
+                # 15 digits of a default Float
+                e = e.xreplace({f: sp.Float(f, 40) for f in e.atoms(sp.Float)})
+            return e
 
         fexpr = [prep(e) for e in f]
         jexpr = [(i, j, prep(J[i, j])) for i in range(n) for j in range(n) if J[i, j] != 0]
@@ -70,9 +95,13 @@ def main():
         rate_slots = sorted({int(str(sym)[5:]) for e in fexpr for sym in e.free_symbols if str(sym).startswith("Rate_")})
         nparam = max([nph] + [k + 1 for k in rate_slots])
         cases = []
-        for case in range(3):
+        designed = gocart_cases(rng, species) if name == "gocart" else None
+        for case in range(len(designed) if designed else 3):
             y = 10.0 ** rng.uniform(6, 12, n)
             T = float(rng.uniform(240.0, 310.0))
+            clw = 0.0
+            if designed:
+                y, T, clw = designed[case]
             rho = float(rng.uniform(1e18, 2.5e19))
             jv = rng.uniform(0.0, 1e-3, nparam)
             if "AIR" in species:
@@ -80,7 +109,8 @@ def main():
             if "M" in species:
                 y[species.index("M")] = rho
             subs = {sp.Symbol("Temp"): sp.Float(T, 40), sp.Symbol("M_density"): sp.Float(rho, 40),
-                    sp.Symbol("AirDensity"): sp.Float(rho, 40)}
+                    sp.Symbol("AirDensity"): sp.Float(rho, 40), sp.Symbol("CLW"): sp.Float(clw, 40),
+                    sp.Symbol("RH"): sp.Float(0.5, 40)}
             for k, s in enumerate(species):
                 subs[sp.Symbol(f"C_{s}")] = sp.Float(float(y[k]), 40)
             for k in range(nph):
@@ -90,8 +120,8 @@ def main():
                 subs[sp.Symbol(f"Rate_{k}")] = sp.Float(float(jv[k]), 40)
             F = [float(sp.N(e.subs(subs), 40)) for e in fexpr]
             Jv = [[i, j, float(sp.N(e.subs(subs), 40))] for i, j, e in jexpr]
-            cases.append({"T": T, "air_density": rho, "y": [float(v) for v in y], "jvals": [float(v) for v in jv],
-                          "F": F, "J": Jv})
+            cases.append({"T": T, "air_density": rho, "clw": clw, "y": [float(v) for v in y],
+                          "jvals": [float(v) for v in jv], "F": F, "J": Jv})
         with open(os.path.join(OUT, f"{name}.json"), "w") as fh:
             json.dump({"mechanism": name, "species": species,
                        "source": "reference src/mkpp/lowering.py symbolic F and J, evaluated at 40 digits",

@@ -15,7 +15,7 @@ from mkpp_amd.gen.model import load_mechanism
 from mkpp_amd.gen.sparsity import build_plan
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "plans")
-MECHS = ["chapman", "small_strato", "carbon", "saprc99", "saprc99_mini", "saprcnov", "ts1"]
+MECHS = ["chapman", "small_strato", "carbon", "saprc99", "saprc99_mini", "saprcnov", "ts1", "gocart"]
 
 
 @pytest.fixture(scope="module", params=MECHS)
