@@ -57,6 +57,8 @@ def main():
                     help="skip the second wave-engine mechanism reported as saprcnov")
     ap.add_argument("--no-ts1", dest="ts1", action="store_false",
                     help="skip the ts1 runner-schedule leg (210 species)")
+    ap.add_argument("--no-gocart", dest="gocart", action="store_false",
+                    help="skip the gocart leg (cloud-gated uptake, NH4/NO3/SO4 equilibrium)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB elementwise copy (x * 1.0) three times: the known-size kernel "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -161,6 +163,8 @@ def main():
     progress("reduction-variant leg done")
     ts1 = run_ts1(np, 1024, 16 if args.cpu_sample else 0, args.cpu_threads) if args.ts1 else None
     progress("ts1 leg done")
+    gocart = run_gocart(np, 65536, 16384 if args.cpu_sample else 0, args.cpu_threads) if args.gocart else None
+    progress("gocart leg done")
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "saprc99_rodas3_262144_traffic.json")
@@ -208,6 +212,7 @@ def main():
         "saprcnov": saprcnov,
         "reduced": reduced,
         "ts1": ts1,
+        "gocart": gocart,
         "configs": configs,
     }
     if cpu:
@@ -270,12 +275,15 @@ def cpu_leg(spec: dict) -> dict:
                 jv[w["photolysis_slots"]] *= scale
         jvs.append(jv)
 
+    cw = w.get("cloud_water")
+
     def run(c, lo, hi, threads):
         for (sec, _), jv in zip(segments, jvs):
             o.integrate(c, sec, w["steps"], w["method"], temp=w["temp"][lo:hi], air_density=w["air_density"][lo:hi],
                         jvals=None if jv is None else np.ascontiguousarray(jv[:, lo:hi]), atol=w["atol"],
                         rtol=w["rtol"], seed=w["seed"], hoist=spec.get("hoist", False),
-                        max_attempts=spec.get("max_attempts", 0), threads=threads)
+                        max_attempts=spec.get("max_attempts", 0), threads=threads,
+                        cloud_water=None if cw is None else np.ascontiguousarray(cw[lo:hi]))
 
     times = []
     for r in range(1 + spec.get("reps", 3)):
@@ -401,6 +409,59 @@ def run_config(np, workload: str, cells: int, cpu_threads: int, reps: int = 5, c
                          "note": f"{cells} cells fill {cells / 128:.0f} workgroups of 128 lanes on 256 CUs: the "
                                  "launch is occupancy- and latency-bound, not VALU-bound"},
             "cpu_baseline": cpu, "gpu_over_cpu": (cells / (ms / 1e3)) / cpu["value"]}
+
+
+def run_gocart(np, cells, cpu_sample, cpu_threads, reps=5):
+    """gocart (26 species, lane engine; cloud-gated uptake and the NH4/NO3/SO4
+    equilibrium block) on workloads.gocart_mixed, Rodas3 1 h rtol 1e-3 atol 1e3,
+    through mkpp_amd_execute with per-cell temperature, density, cloud water
+    and forcing in device memory; HIP events around each call on the launch
+    stream (the call synchronizes it), the median of `reps`."""
+    import torch
+
+    from mkpp_amd import native, workloads
+
+    w = workloads.gocart_mixed(cells)
+    plan = json.load(open(os.path.join(ROOT, "mkpp_amd", "plans", "gocart.json")))
+    step_flops = plan["step_flops"]["rodas3"]
+    dev = torch.device("cuda:0")
+    init = torch.tensor(w["conc"], device=dev).contiguous()
+    rp = torch.tensor(w["rate_parameters"], device=dev).contiguous()
+    T = torch.tensor(w["temp"], device=dev)
+    rho = torch.tensor(w["air_density"], device=dev)
+    cw = torch.tensor(w["cloud_water"], device=dev)
+    h = native.Handle("gocart", cells)
+    h.set_solver("rodas3", w["atol"], w["rtol"], w["seed"])
+    stream = torch.cuda.current_stream()
+
+    def launch(buf):
+        h.execute(buf, w["dt"], 1, temp=T, air_density=rho, cloud_water=cw, rate_parameters=rp, stream=stream)
+
+    launch(init.clone())
+    ms_all, att = [], []
+    for _ in range(reps):
+        buf = init.clone()
+        ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        ev[0].record(stream)
+        launch(buf)
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        ms_all.append(ev[0].elapsed_time(ev[1]))
+        st, _, a = h.cell_status()
+        att.append(a.astype(np.float64))
+    ms = float(np.median(ms_all))
+    attempted = np.mean(att, axis=0)
+    achieved = float(attempted.sum()) * step_flops / (ms / 1e3)
+    cpu = None
+    if cpu_sample > 0:
+        cpu = cpu_baseline({"workload": "gocart_mixed", "cells": cells, "sample": min(cpu_sample, cells)}, cpu_threads)
+    h.close()
+    return {"workload": f"gocart Rodas3 1 h rtol 1e-3 atol 1e3, {cells} cells (gocart_mixed: sulfate ratios "
+                        "0.2-20, cloud water around the 1e-6 gate)", "value": cells / (ms / 1e3),
+            "unit": "grid-cells/s", "kernel_ms": ms, "attempted_steps_mean": float(attempted.mean()),
+            "status_nonzero": int((st != 0).sum()), "fp64_tflops": achieved / 1e12,
+            "roofline_frac": achieved / PEAK_FP64, "flops_per_attempted_step": step_flops, "cpu_baseline": cpu,
+            "gpu_over_cpu": (cells / (ms / 1e3)) / cpu["value"] if cpu else None}
 
 
 def run_saprcnov(np, cells, cpu_sample, cpu_threads, reps=2, max_attempts=5000):

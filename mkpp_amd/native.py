@@ -166,6 +166,16 @@ def mechanism_info(name: str) -> dict:
     return info
 
 
+class ExecutionParams(C.Structure):
+    """mkpp_amd_execution_params (include/mkpp_amd.h)."""
+
+    _fields_ = [("dt", C.c_double), ("steps", C.c_int), ("jvals", C.c_void_p), ("jvals_extent", C.c_long),
+                ("temp", C.c_double), ("air_density", C.c_double), ("rh", C.c_double), ("clw", C.c_double),
+                ("cell_temperature", C.c_void_p), ("cell_humidity", C.c_void_p), ("cell_density", C.c_void_p),
+                ("cell_cloud_water", C.c_void_p), ("cell_jvals", C.c_void_p), ("cell_jvals_extent0", C.c_long),
+                ("cell_jvals_extent1", C.c_long), ("cell_jvals_stride0", C.c_long), ("cell_jvals_stride1", C.c_long)]
+
+
 class Handle:
     """Handle API over host numpy buffers (borrowed, like the reference's C ABI)."""
 
@@ -229,6 +239,31 @@ class Handle:
 
     def integrate(self, dt: float, steps: int = 1):
         self._check(lib().mkpp_amd_integrate_steps(self._h, float(dt), int(steps)), "integrate")
+
+    def execute(self, conc, dt: float, steps: int = 1, temp=None, air_density=None, cloud_water=None,
+                rate_parameters=None, stream=None):
+        """mkpp_amd_execute on device-resident cells (torch CUDA tensors,
+        float64): conc (N, cells) contiguous; temp, air_density, cloud_water
+        per cell or None (the execution defaults 288.15 K, 2.4476e19, 0);
+        rate_parameters (P, cells) contiguous or None.  Enqueued on `stream`
+        (default: torch's current stream) and synchronous, with the handle's
+        solver options; per-cell outcomes via cell_status()."""
+        import torch
+
+        n = self.num_cells
+        assert tuple(conc.shape) == (self.info["num_species"], n) and conc.is_contiguous()
+        p = ExecutionParams(dt=float(dt), steps=int(steps), temp=288.15, air_density=2.4476e19, rh=0.5, clw=0.0)
+        for field, t in (("cell_temperature", temp), ("cell_density", air_density), ("cell_cloud_water", cloud_water)):
+            if t is not None:
+                assert tuple(t.shape) == (n,) and t.is_contiguous()
+                setattr(p, field, t.data_ptr())
+        if rate_parameters is not None:
+            assert rate_parameters.shape[1] == n and rate_parameters.is_contiguous()
+            p.cell_jvals = rate_parameters.data_ptr()
+            p.cell_jvals_extent0, p.cell_jvals_extent1 = n, rate_parameters.shape[0]
+            p.cell_jvals_stride0, p.cell_jvals_stride1 = 1, n
+        s = (torch.cuda.current_stream() if stream is None else stream).cuda_stream
+        self._check(lib().mkpp_amd_execute(self._h, conc.data_ptr(), n, n, C.addressof(p), s), "execute")
 
     def cell_status(self):
         st = np.zeros(self.num_cells, np.int32)

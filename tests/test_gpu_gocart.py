@@ -21,7 +21,6 @@ ratios 0.2..20, cloud water 0 or 1e-7..1e-5 around the 1e-6 gate).
   handle's integrate on host buffers bit for bit.
 """
 
-import ctypes as C
 import json
 import os
 
@@ -145,14 +144,6 @@ def test_gocart_integrate_matches_oracle(cells, method):
     assert same.mean() >= 0.9 and (r <= bar).all()
 
 
-class _Exec(C.Structure):
-    _fields_ = [("dt", C.c_double), ("steps", C.c_int), ("jvals", C.c_void_p), ("jvals_extent", C.c_long),
-                ("temp", C.c_double), ("air_density", C.c_double), ("rh", C.c_double), ("clw", C.c_double),
-                ("cell_temperature", C.c_void_p), ("cell_humidity", C.c_void_p), ("cell_density", C.c_void_p),
-                ("cell_cloud_water", C.c_void_p), ("cell_jvals", C.c_void_p), ("cell_jvals_extent0", C.c_long),
-                ("cell_jvals_extent1", C.c_long), ("cell_jvals_stride0", C.c_long), ("cell_jvals_stride1", C.c_long)]
-
-
 def test_gocart_execute_with_cell_cloud_water_equals_handle(cells):
     w = _subset(cells, 1000)
     h, ref = _handle(w)
@@ -166,15 +157,10 @@ def test_gocart_execute_with_cell_cloud_water_equals_handle(cells):
     T = torch.tensor(w["temp"], device=dev)
     rho = torch.tensor(w["air_density"], device=dev)
     clw = torch.tensor(w["cloud_water"], device=dev)
-    jv = torch.tensor(w["rate_parameters"], device=dev).contiguous()  # [1 x n]: cell c's slot p at c + n p
-    p = _Exec(dt=w["dt"], steps=1, temp=288.15, air_density=2.4476e19, rh=0.5, clw=0.0,
-              cell_temperature=T.data_ptr(), cell_density=rho.data_ptr(), cell_cloud_water=clw.data_ptr(),
-              cell_jvals=jv.data_ptr(), cell_jvals_extent0=n, cell_jvals_extent1=1, cell_jvals_stride0=1,
-              cell_jvals_stride1=n)
+    jv = torch.tensor(w["rate_parameters"], device=dev).contiguous()
     h2 = native.Handle(MECH, n)
     h2.set_solver("rodas3", w["atol"], w["rtol"], w["seed"])
-    torch.cuda.synchronize()
-    h2._check(native.lib().mkpp_amd_execute(h2._h, conc.data_ptr(), n, n, C.addressof(p), None), "execute")
+    h2.execute(conc, w["dt"], 1, temp=T, air_density=rho, cloud_water=clw, rate_parameters=jv)
     assert (h2.cell_status()[0] == 0).all()
     got = conc.cpu().numpy()
     assert np.array_equal(got, ref)
