@@ -105,32 +105,43 @@ struct Code {
     if (__builtin_amdgcn_inverse_ballot_w64(lane_mask<0x00009000u, 0x00040000u, 27>(zs))) lu[rb[0] + 10] = c10;
   }
 
-  // Solves: every lane loads its entry (in bounds for any row and column),
-  // zeroed outside the column's rows by a factor off the substitution chain.
+  // Solves: every lane loads its entry (in bounds for any row and column) a
+  // chunk of terms ahead, and a select on the column's constant lane mask (an
+  // SGPR pair) zeroes it outside the column's rows, off the substitution chain.
   // Forward substitution, column oriented, in level order: x_k = w_k is final
   // when its level is reached; rows of L(:, k) subtract L(i,k) x_k.
   static __device__ __forceinline__ void forward(const double* __restrict__ lu, const int (&rb)[1], double (&wv)[1], int lane) {
+    (void)lane;
+    double pf[5];  // factor entries, loaded 6 to 12 terms ahead of their update
+    int zs = 0;  // laundered once per call: see lane_mask
+    asm volatile("" : "+s"(zs));
+    pf[0] = lu[rb[0] + 2];
+    pf[1] = lu[rb[0] + 3];
+    pf[2] = lu[rb[0] + 4];
+    pf[3] = lu[rb[0] + 6];
+    pf[4] = lu[rb[0] + 8];
+    __builtin_amdgcn_sched_barrier(0);
     {  // level 0: pivots 2, 3
       const double x2 = readlane_d(wv[0], 35);
       const double x3 = readlane_d(wv[0], 24);
-      { const double l = lu[rb[0] + 2] * ((lane == 0) ? 1.0 : 0.0);
+      { const double l = lane_keep(lane_mask<0x00000001u, 0x00000000u, 1>(zs), pf[0]);
         wv[0] = fma(-l, x2, wv[0]); }
-      { const double l = lu[rb[0] + 3] * ((lane == 0) ? 1.0 : 0.0);
+      { const double l = lane_keep(lane_mask<0x00000001u, 0x00000000u, 2>(zs), pf[1]);
         wv[0] = fma(-l, x3, wv[0]); }
     }
     {  // level 1: pivots 4
       const double x4 = readlane_d(wv[0], 0);
-      { const double l = lu[rb[0] + 4] * (((0x0000000000000048ull >> lane) & 1ull) ? 1.0 : 0.0);
+      { const double l = lane_keep(lane_mask<0x00000048u, 0x00000000u, 3>(zs), pf[2]);
         wv[0] = fma(-l, x4, wv[0]); }
     }
     {  // level 2: pivots 6
       const double x6 = readlane_d(wv[0], 6);
-      { const double l = lu[rb[0] + 6] * (((0x0000000000001200ull >> lane) & 1ull) ? 1.0 : 0.0);
+      { const double l = lane_keep(lane_mask<0x00001200u, 0x00000000u, 4>(zs), pf[3]);
         wv[0] = fma(-l, x6, wv[0]); }
     }
     {  // level 3: pivots 8
       const double x8 = readlane_d(wv[0], 12);
-      { const double l = lu[rb[0] + 8] * (((0x0004000000008000ull >> lane) & 1ull) ? 1.0 : 0.0);
+      { const double l = lane_keep(lane_mask<0x00008000u, 0x00040000u, 5>(zs), pf[4]);
         wv[0] = fma(-l, x8, wv[0]); }
     }
   }
@@ -140,25 +151,35 @@ struct Code {
   static __device__ __forceinline__ void backward(const double* __restrict__ lu, const int (&rb)[1], double (&wv)[1], const double (&rdiag)[1], double (&x)[1], int lane) {
     // w_k is final once column k is reached (later columns update rows above
     // it only), so x = w * (1 / U(k,k)) is formed for every row at the end.
+    (void)lane;
+    double pf[5];  // factor entries, loaded 6 to 12 terms ahead of their update
+    int zs = 0;  // laundered once per call: see lane_mask
+    asm volatile("" : "+s"(zs));
+    pf[0] = lu[rb[0] + 10];
+    pf[1] = lu[rb[0] + 3];
+    pf[2] = lu[rb[0] + 2];
+    pf[3] = lu[rb[0] + 8];
+    pf[4] = lu[rb[0] + 6];
+    __builtin_amdgcn_sched_barrier(0);
     {  // level 0: pivots 10, 3, 2
       const double x10 = readlane_d(wv[0] * rdiag[0], 50);
       const double x3 = readlane_d(wv[0] * rdiag[0], 24);
       const double x2 = readlane_d(wv[0] * rdiag[0], 35);
-      { const double u = lu[rb[0] + 10] * (((0x0000000000009000ull >> lane) & 1ull) ? 1.0 : 0.0);
+      { const double u = lane_keep(lane_mask<0x00009000u, 0x00000000u, 1>(zs), pf[0]);
         wv[0] = fma(-u, x10, wv[0]); }
-      { const double u = lu[rb[0] + 3] * ((lane == 18) ? 1.0 : 0.0);
+      { const double u = lane_keep(lane_mask<0x00040000u, 0x00000000u, 2>(zs), pf[1]);
         wv[0] = fma(-u, x3, wv[0]); }
-      { const double u = lu[rb[0] + 2] * ((lane == 11) ? 1.0 : 0.0);
+      { const double u = lane_keep(lane_mask<0x00000800u, 0x00000000u, 3>(zs), pf[2]);
         wv[0] = fma(-u, x2, wv[0]); }
     }
     {  // level 1: pivots 8
       const double x8 = readlane_d(wv[0] * rdiag[0], 12);
-      { const double u = lu[rb[0] + 8] * (((0x0000000000000240ull >> lane) & 1ull) ? 1.0 : 0.0);
+      { const double u = lane_keep(lane_mask<0x00000240u, 0x00000000u, 4>(zs), pf[3]);
         wv[0] = fma(-u, x8, wv[0]); }
     }
     {  // level 2: pivots 6
       const double x6 = readlane_d(wv[0] * rdiag[0], 6);
-      { const double u = lu[rb[0] + 6] * (((0x0000000000000009ull >> lane) & 1ull) ? 1.0 : 0.0);
+      { const double u = lane_keep(lane_mask<0x00000009u, 0x00000000u, 5>(zs), pf[4]);
         wv[0] = fma(-u, x6, wv[0]); }
     }
     x[0] = wv[0] * rdiag[0];

@@ -71,19 +71,27 @@ struct Code {
     if (__builtin_amdgcn_inverse_ballot_w64(lane_mask<0x00000049u, 0x00000000u, 8>(zs))) lu[rb[0] + 2] = c2;
   }
 
-  // Solves: every lane loads its entry (in bounds for any row and column),
-  // zeroed outside the column's rows by a factor off the substitution chain.
+  // Solves: every lane loads its entry (in bounds for any row and column) a
+  // chunk of terms ahead, and a select on the column's constant lane mask (an
+  // SGPR pair) zeroes it outside the column's rows, off the substitution chain.
   // Forward substitution, column oriented, in level order: x_k = w_k is final
   // when its level is reached; rows of L(:, k) subtract L(i,k) x_k.
   static __device__ __forceinline__ void forward(const double* __restrict__ lu, const int (&rb)[1], double (&wv)[1], int lane) {
+    (void)lane;
+    double pf[2];  // factor entries, loaded 6 to 12 terms ahead of their update
+    int zs = 0;  // laundered once per call: see lane_mask
+    asm volatile("" : "+s"(zs));
+    pf[0] = lu[rb[0] + 0];
+    pf[1] = lu[rb[0] + 1];
+    __builtin_amdgcn_sched_barrier(0);
     {  // level 0: pivots 0
       const double x0 = readlane_d(wv[0], 0);
-      { const double l = lu[rb[0] + 0] * (((0x0000000000000048ull >> lane) & 1ull) ? 1.0 : 0.0);
+      { const double l = lane_keep(lane_mask<0x00000048u, 0x00000000u, 1>(zs), pf[0]);
         wv[0] = fma(-l, x0, wv[0]); }
     }
     {  // level 1: pivots 1
       const double x1 = readlane_d(wv[0], 3);
-      { const double l = lu[rb[0] + 1] * ((lane == 6) ? 1.0 : 0.0);
+      { const double l = lane_keep(lane_mask<0x00000040u, 0x00000000u, 2>(zs), pf[1]);
         wv[0] = fma(-l, x1, wv[0]); }
     }
   }
@@ -93,14 +101,21 @@ struct Code {
   static __device__ __forceinline__ void backward(const double* __restrict__ lu, const int (&rb)[1], double (&wv)[1], const double (&rdiag)[1], double (&x)[1], int lane) {
     // w_k is final once column k is reached (later columns update rows above
     // it only), so x = w * (1 / U(k,k)) is formed for every row at the end.
+    (void)lane;
+    double pf[2];  // factor entries, loaded 6 to 12 terms ahead of their update
+    int zs = 0;  // laundered once per call: see lane_mask
+    asm volatile("" : "+s"(zs));
+    pf[0] = lu[rb[0] + 2];
+    pf[1] = lu[rb[0] + 1];
+    __builtin_amdgcn_sched_barrier(0);
     {  // level 0: pivots 2
       const double x2 = readlane_d(wv[0] * rdiag[0], 6);
-      { const double u = lu[rb[0] + 2] * (((0x0000000000000009ull >> lane) & 1ull) ? 1.0 : 0.0);
+      { const double u = lane_keep(lane_mask<0x00000009u, 0x00000000u, 1>(zs), pf[0]);
         wv[0] = fma(-u, x2, wv[0]); }
     }
     {  // level 1: pivots 1
       const double x1 = readlane_d(wv[0] * rdiag[0], 3);
-      { const double u = lu[rb[0] + 1] * ((lane == 0) ? 1.0 : 0.0);
+      { const double u = lane_keep(lane_mask<0x00000001u, 0x00000000u, 2>(zs), pf[1]);
         wv[0] = fma(-u, x1, wv[0]); }
     }
     x[0] = wv[0] * rdiag[0];

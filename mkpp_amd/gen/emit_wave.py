@@ -501,6 +501,9 @@ def write(mech: Mechanism, p: dict, outdir: str) -> list:
 REGISTER_BLOCK = 35
 # Pivot-row broadcasts of the register block issued together (see factor).
 BROADCAST_GROUP = 4
+# Solve factor entries are loaded in chunks of this many terms, one chunk
+# ahead of the updates (6 timed fastest of 4-12 on the saprc99 Rodas3 cells).
+SOLVE_PREFETCH = 6
 
 
 def _lane_test(mask: int, var: str = "lane") -> str:
@@ -877,11 +880,12 @@ def emit_code(t: dict, ct: dict) -> str:
     w("  }")
     w("")
     # ---- forward and backward substitution ----
-    # The solves load their factor entry in every lane, scale it by 1 in the
-    # column's rows and by 0 elsewhere (a select of constants, not a branch)
-    # and update every lane: fma(-0, x, w) = w for finite x, and the loads and
-    # scalings, which depend on nothing, run ahead of the substitution chain.  Every such address, R[row] + k or
-    # R[k] + row with row, k < NV, lies inside the wavefront's LDS block.
+    # The solves load their factor entry in every lane, keep it in the
+    # column's rows and put 0 elsewhere (a select on a constant SGPR-pair lane
+    # mask, not a branch) and update every lane: fma(-0, x, w) = w for finite
+    # x, and the loads and selects, which depend on nothing, run ahead of the
+    # substitution chain.  Every such address, R[row] + k or R[k] + row with
+    # row, k < NV, lies inside the wavefront's LDS block.
     lds_doubles = t["lds_doubles"]
     assert min(R) >= 0 and max(R) + nv - 1 < lds_doubles
     def emit_levels(level, order, masks_fn, value_fn, name, addr_fn, acc):
@@ -892,7 +896,31 @@ def emit_code(t: dict, ct: dict) -> str:
         pivots keep `order`; a row therefore takes its updates in (level, k)
         order."""
         piv = [k for k in order if any(masks_fn(k))]
-        for lv in sorted(set(level[k] for k in piv)):
+        levels = sorted(set(level[k] for k in piv))
+        terms = [(k, b) for lv in levels for k in piv if level[k] == lv for b in range(nb) if masks_fn(k)[b]]
+        nt, d = len(terms), SOLVE_PREFETCH
+        sites = _MaskSites()
+        w("    (void)lane;")
+        if nt == 0:
+            w("    (void)lu; (void)rb;")
+            return
+        w(f"    double pf[{nt}];  // factor entries, loaded {d} to {2 * d} terms ahead of their update")
+        w(_OPAQUE_SGPR)
+
+        def issue(c, ind):
+            # chunk c of the loads; the barrier keeps them ahead of the updates
+            # of chunk c - 1 (the scheduler would otherwise sink each load to
+            # its use and expose the LDS latency on every term)
+            if c * d >= nt:
+                return
+            for t in range(c * d, min((c + 1) * d, nt)):
+                k, b = terms[t]
+                w(f"{ind}pf[{t}] = {addr_fn(k, b)};")
+            w(f"{ind}__builtin_amdgcn_sched_barrier(0);")
+
+        issue(0, "    ")
+        t = 0
+        for lv in levels:
             ks = [k for k in piv if level[k] == lv]
             w(f"    {{  // level {lv}: pivots {', '.join(str(k) for k in ks)}")
             for k in ks:
@@ -901,12 +929,16 @@ def emit_code(t: dict, ct: dict) -> str:
                 masks = masks_fn(k)
                 for b in range(nb):
                     if masks[b]:
-                        w(f"      {{ const double {name} = {addr_fn(k, b)} * ({_lane_test(masks[b])} ? 1.0 : 0.0);")
+                        if t % d == 0:
+                            issue(t // d + 1, "      ")
+                        w(f"      {{ const double {name} = {sites.keep(masks[b], f'pf[{t}]')};")
                         w(f"        {acc}[{b}] = fma(-{name}, x{k}, {acc}[{b}]); }}")
+                        t += 1
             w("    }")
 
-    w("  // Solves: every lane loads its entry (in bounds for any row and column),")
-    w("  // zeroed outside the column's rows by a factor off the substitution chain.")
+    w("  // Solves: every lane loads its entry (in bounds for any row and column) a")
+    w("  // chunk of terms ahead, and a select on the column's constant lane mask (an")
+    w("  // SGPR pair) zeroes it outside the column's rows, off the substitution chain.")
     w("  // Forward substitution, column oriented, in level order: x_k = w_k is final")
     w("  // when its level is reached; rows of L(:, k) subtract L(i,k) x_k.")
     w("  static __device__ __forceinline__ void forward(const double* __restrict__ lu, const int (&rb)[" + str(nb) + "], double (&wv)[" + str(nb) + "], int lane) {")
