@@ -504,6 +504,12 @@ BROADCAST_GROUP = 4
 # Solve factor entries are loaded in chunks of this many terms, one chunk
 # ahead of the updates (6 timed fastest of 4-12 on the saprc99 Rodas3 cells).
 SOLVE_PREFETCH = 6
+# Trailing-block columns held in registers through the LDS-phase pivots, the
+# ones those pivots update most, per mechanism.  Timed on the bench cells:
+# saprc99 Rodas3 120.7 -> 114.4 ms with 12 (8: 115.0; 13 or more put the
+# kernel past its SGPR budget and spill masks to VGPR lanes: 126 ms);
+# saprcnov 449 -> 432 ms with 12.
+LDS_PHASE_REGISTER_COLUMNS = {"saprc99": 12, "saprcnov": 12}
 
 
 def _lane_test(mask: int, var: str = "lane") -> str:
@@ -821,6 +827,26 @@ def emit_code(t: dict, ct: dict) -> str:
     w(_OPAQUE_SGPR)
     ms = _MaskSites()
     k0 = t["k0"]
+    pattern = set(Lset) | set(Uset) | {(i, i) for i in range(nv)}
+    # Columns of the trailing block that the LDS-phase pivots update most are
+    # held in registers through that phase (e<j>_<bank>, lane = row), so that
+    # their updates chain through registers and pivot rows read them with
+    # readlane instead of an LDS store-to-load round trip.
+    upd = {}
+    for k in range(k0):
+        nrow = t["lofs"][k + 1] - t["lofs"][k]
+        for j in t["ucol"][t["uofs"][k]:t["uofs"][k + 1]]:
+            upd[j] = upd.get(j, 0) + nrow
+    nrc = LDS_PHASE_REGISTER_COLUMNS.get(name, 0)
+    rc = sorted(sorted((j for j in upd if j >= k0), key=lambda j: (-upd[j], j))[:nrc])
+    rcb = {j: [b for b in range(nb) if masks_of([r for r in range(nv) if (r, j) in pattern])[b]] for j in rc}
+    if rc:
+        w(f"    // columns {', '.join(str(j) for j in rc)} in registers through the pivots below")
+    for j in rc:
+        # unmasked: lanes outside the column's rows are never updated, read or
+        # stored (a plain load leaves the scheduler free to issue them early)
+        for b in rcb[j]:
+            w(f"    double e{j}_{b} = lu[rb[{b}] + {j}];")
     for k in range(k0):
         rows = t["lrow"][t["lofs"][k]:t["lofs"][k + 1]]
         if not rows:
@@ -829,7 +855,10 @@ def emit_code(t: dict, ct: dict) -> str:
         w(f"    {{  // pivot {k}: {len(rows)} rows x {len(cols)} columns")
         w(f"      const double d = pv[{R[k] + k}];")
         for j in cols:
-            w(f"      const double u{j} = pv[{R[k] + j}];")
+            if j in rc:
+                w(f"      const double u{j} = readlane_d(e{j}_{slot[k] // 64}, {slot[k] % 64});")
+            else:
+                w(f"      const double u{j} = pv[{R[k] + j}];")
         fm = masks_of(rows)
         for b in range(nb):
             m = fm[b]
@@ -840,20 +869,28 @@ def emit_code(t: dict, ct: dict) -> str:
             w(f"        const double l = r[{k}] / d;")
             w(f"        r[{k}] = l;")
             for j in cols:
-                w(f"        r[{j}] = fma(-l, u{j}, r[{j}]);")
+                if j in rc:
+                    w(f"        e{j}_{b} = fma(-l, u{j}, e{j}_{b});")
+                else:
+                    w(f"        r[{j}] = fma(-l, u{j}, r[{j}]);")
             w("      }")
         w("    }")
+    for j in rc:
+        # rows above the trailing block are final; the block's rows go on in c<j>
+        for b in rcb[j]:
+            m = masks_of([r for r in range(nv) if (r, j) in pattern and r < k0])[b]
+            if m:
+                w(f"    if ({ms.test(m)}) lu[rb[{b}] + {j}] = e{j}_{b};")
     if k0 < nv:
         # trailing block: rows and columns >= k0, all rows in bank 0
         assert all(slot[r] < 64 for r in range(k0, nv))
-        pattern = set(Lset) | set(Uset) | {(i, i) for i in range(nv)}
         w(f"    // Rows and columns {k0}..{nv - 1} in registers: lane = row (bank 0), c<j> =")
         w("    // the row's entry in column j (0 where the pattern has none).  Pivot rows")
         w("    // are broadcast with readlane; the multiplier is zeroed outside L(k), so")
         w("    // every update is an unconditional fma that leaves other lanes unchanged.")
         for c in range(k0, nv):
             m = masks_of([r for r in range(k0, nv) if (r, c) in pattern])[0]
-            w(f"    double c{c} = {ms.keep(m, f'lu[rb[0] + {c}]')};")
+            w(f"    double c{c} = {ms.keep(m, f'e{c}_0' if c in rc else f'lu[rb[0] + {c}]')};")
         for k in range(k0, nv):
             rows = t["lrow"][t["lofs"][k]:t["lofs"][k + 1]]
             if not rows:
@@ -888,21 +925,16 @@ def emit_code(t: dict, ct: dict) -> str:
     # row, k < NV, lies inside the wavefront's LDS block.
     lds_doubles = t["lds_doubles"]
     assert min(R) >= 0 and max(R) + nv - 1 < lds_doubles
-    def emit_levels(level, order, masks_fn, value_fn, name, addr_fn, acc):
-        """Column-oriented substitution in level order: the pivots of one level
-        depend only on earlier levels, so their broadcasts (readlane) are all
-        issued before the level's updates, and the substitution chain runs
-        through the levels rather than through every pivot.  Within a level,
-        pivots keep `order`; a row therefore takes its updates in (level, k)
-        order."""
-        piv = [k for k in order if any(masks_fn(k))]
-        levels = sorted(set(level[k] for k in piv))
-        terms = [(k, b) for lv in levels for k in piv if level[k] == lv for b in range(nb) if masks_fn(k)[b]]
+    def emit_terms(groups, name, bases):
+        """Updates acc[b] -= v * x over `groups` of (comment, broadcast lines,
+        terms (addr, mask, bank, x)), in order.  Every term's factor entry is
+        loaded a chunk ahead (`pf`) and kept in the lanes of its mask."""
+        terms = [tm for (_, _, tms) in groups for tm in tms]
         nt, d = len(terms), SOLVE_PREFETCH
         sites = _MaskSites()
         w("    (void)lane;")
         if nt == 0:
-            w("    (void)lu; (void)rb;")
+            w(f"    (void)lu; (void){bases};")
             return
         w(f"    double pf[{nt}];  // factor entries, loaded {d} to {2 * d} terms ahead of their update")
         w(_OPAQUE_SGPR)
@@ -913,28 +945,39 @@ def emit_code(t: dict, ct: dict) -> str:
             # its use and expose the LDS latency on every term)
             if c * d >= nt:
                 return
-            for t in range(c * d, min((c + 1) * d, nt)):
-                k, b = terms[t]
-                w(f"{ind}pf[{t}] = {addr_fn(k, b)};")
+            for q in range(c * d, min((c + 1) * d, nt)):
+                w(f"{ind}pf[{q}] = {terms[q][0]};")
             w(f"{ind}__builtin_amdgcn_sched_barrier(0);")
 
         issue(0, "    ")
-        t = 0
-        for lv in levels:
-            ks = [k for k in piv if level[k] == lv]
-            w(f"    {{  // level {lv}: pivots {', '.join(str(k) for k in ks)}")
-            for k in ks:
-                w(f"      const double x{k} = readlane_d({value_fn(k)}, {slot[k] % 64});")
-            for k in ks:
-                masks = masks_fn(k)
-                for b in range(nb):
-                    if masks[b]:
-                        if t % d == 0:
-                            issue(t // d + 1, "      ")
-                        w(f"      {{ const double {name} = {sites.keep(masks[b], f'pf[{t}]')};")
-                        w(f"        {acc}[{b}] = fma(-{name}, x{k}, {acc}[{b}]); }}")
-                        t += 1
+        q = 0
+        for comment, head, tms in groups:
+            w(f"    {{  // {comment}")
+            for line in head:
+                w(f"      {line}")
+            for addr, m, b, x, acc in tms:
+                if q % d == 0:
+                    issue(q // d + 1, "      ")
+                w(f"      {{ const double {name} = {sites.keep(m, f'pf[{q}]')};")
+                w(f"        {acc}[{b}] = fma(-{name}, {x}, {acc}[{b}]); }}")
+                q += 1
             w("    }")
+
+    def emit_levels(level, order, masks_fn, value_fn, name, addr_fn, acc, bases="rb"):
+        """Column-oriented substitution in level order: the pivots of one level
+        depend only on earlier levels, so their broadcasts (readlane) are all
+        issued before the level's updates, and the substitution chain runs
+        through the levels rather than through every pivot.  Within a level,
+        pivots keep `order`; a row therefore takes its updates in (level, k)
+        order."""
+        piv = [k for k in order if any(masks_fn(k))]
+        groups = []
+        for lv in sorted(set(level[k] for k in piv)):
+            ks = [k for k in piv if level[k] == lv]
+            head = [f"const double x{k} = readlane_d({value_fn(k)}, {slot[k] % 64});" for k in ks]
+            tms = [(addr_fn(k, b), masks_fn(k)[b], b, f"x{k}", acc) for k in ks for b in range(nb) if masks_fn(k)[b]]
+            groups.append((f"level {lv}: pivots {', '.join(str(k) for k in ks)}", head, tms))
+        emit_terms(groups, name, bases)
 
     w("  // Solves: every lane loads its entry (in bounds for any row and column) a")
     w("  // chunk of terms ahead, and a select on the column's constant lane mask (an")
@@ -974,6 +1017,9 @@ def emit_code(t: dict, ct: dict) -> str:
     w("  static __device__ __forceinline__ void tforward(const double* __restrict__ lu, const int (&rv)[" + str(nb) + "], double (&b)[" + str(nb) + "], const double (&rdiag)[" + str(nb) + "], double (&z)[" + str(nb) + "], int lane) {")
     w("    // b_k is final once column k is reached, so z = b * (1 / U(k,k)) is formed")
     w("    // for every row at the end.")
+    # One pivot after another, each factor entry scaled by a computed 0/1
+    # lane factor.  (Level order with chunked loads and SGPR-pair masks, as in
+    # the forward solves, timed slower in the adjoint sweep: 87.4 -> 89.8 ms.)
     for k in range(nv):
         masks = masks_of([j for (kk, j) in Uset if kk == k and j > k])
         if not any(masks):
