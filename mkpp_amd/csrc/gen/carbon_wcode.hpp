@@ -185,77 +185,76 @@ struct Code {
     x[0] = wv[0] * rdiag[0];
   }
 
-  // U^T z = b, column oriented over U's rows: z_k = b_k / U(k,k) once column k
-  // is reached, then every j > k in row k of U subtracts U(k,j) z_k.  The lane
-  // holding row j (rv) reads U(k,j) at R[k] + j.
+  // U^T z = b, column oriented over U's rows in level order: z_k = b_k / U(k,k)
+  // once column k is reached, then every j > k in row k of U subtracts
+  // U(k,j) z_k.  The lane holding row j (rv) reads U(k,j) at R[k] + j.
   static __device__ __forceinline__ void tforward(const double* __restrict__ lu, const int (&rv)[1], double (&b)[1], const double (&rdiag)[1], double (&z)[1], int lane) {
     // b_k is final once column k is reached, so z = b * (1 / U(k,k)) is formed
     // for every row at the end.
-    { const double zk = readlane_d(b[0] * rdiag[0], 11);
+    {  // level 0: pivots 0, 1, 4, 5, 7, 9
+      const double x0 = readlane_d(b[0] * rdiag[0], 11);
+      const double x1 = readlane_d(b[0] * rdiag[0], 18);
+      const double x4 = readlane_d(b[0] * rdiag[0], 0);
+      const double x5 = readlane_d(b[0] * rdiag[0], 3);
+      const double x7 = readlane_d(b[0] * rdiag[0], 9);
+      const double x9 = readlane_d(b[0] * rdiag[0], 15);
       { const double u = lu[11 + rv[0]] * ((lane == 35) ? 1.0 : 0.0);
-        b[0] = fma(-u, zk, b[0]); }
-    }
-    { const double zk = readlane_d(b[0] * rdiag[0], 18);
+        b[0] = fma(-u, x0, b[0]); }
       { const double u = lu[18 + rv[0]] * ((lane == 24) ? 1.0 : 0.0);
-        b[0] = fma(-u, zk, b[0]); }
-    }
-    { const double zk = readlane_d(b[0] * rdiag[0], 0);
+        b[0] = fma(-u, x1, b[0]); }
       { const double u = lu[0 + rv[0]] * ((lane == 6) ? 1.0 : 0.0);
-        b[0] = fma(-u, zk, b[0]); }
-    }
-    { const double zk = readlane_d(b[0] * rdiag[0], 3);
+        b[0] = fma(-u, x4, b[0]); }
       { const double u = lu[3 + rv[0]] * ((lane == 6) ? 1.0 : 0.0);
-        b[0] = fma(-u, zk, b[0]); }
-    }
-    { const double zk = readlane_d(b[0] * rdiag[0], 6);
-      { const double u = lu[6 + rv[0]] * ((lane == 12) ? 1.0 : 0.0);
-        b[0] = fma(-u, zk, b[0]); }
-    }
-    { const double zk = readlane_d(b[0] * rdiag[0], 9);
+        b[0] = fma(-u, x5, b[0]); }
       { const double u = lu[9 + rv[0]] * ((lane == 12) ? 1.0 : 0.0);
-        b[0] = fma(-u, zk, b[0]); }
-    }
-    { const double zk = readlane_d(b[0] * rdiag[0], 12);
-      { const double u = lu[12 + rv[0]] * ((lane == 50) ? 1.0 : 0.0);
-        b[0] = fma(-u, zk, b[0]); }
-    }
-    { const double zk = readlane_d(b[0] * rdiag[0], 15);
+        b[0] = fma(-u, x7, b[0]); }
       { const double u = lu[15 + rv[0]] * ((lane == 50) ? 1.0 : 0.0);
-        b[0] = fma(-u, zk, b[0]); }
+        b[0] = fma(-u, x9, b[0]); }
+    }
+    {  // level 1: pivots 6
+      const double x6 = readlane_d(b[0] * rdiag[0], 6);
+      { const double u = lu[6 + rv[0]] * ((lane == 12) ? 1.0 : 0.0);
+        b[0] = fma(-u, x6, b[0]); }
+    }
+    {  // level 2: pivots 8
+      const double x8 = readlane_d(b[0] * rdiag[0], 12);
+      { const double u = lu[12 + rv[0]] * ((lane == 50) ? 1.0 : 0.0);
+        b[0] = fma(-u, x8, b[0]); }
     }
     z[0] = b[0] * rdiag[0];
   }
 
-  // L^T x = z in place (unit diagonal): x_k = z_k when column k is reached,
-  // descending; every j < k in row k of L subtracts L(k,j) x_k.
+  // L^T x = z in place (unit diagonal), in descending level order: x_k = z_k
+  // when column k is reached; every j < k in row k of L subtracts L(k,j) x_k.
   static __device__ __forceinline__ void tbackward(const double* __restrict__ lu, const int (&rv)[1], double (&z)[1], int lane) {
-    { const double xk = readlane_d(z[0], 50);
+    {  // level 0: pivots 10, 9, 7, 5
+      const double x10 = readlane_d(z[0], 50);
+      const double x9 = readlane_d(z[0], 15);
+      const double x7 = readlane_d(z[0], 9);
+      const double x5 = readlane_d(z[0], 3);
       { const double l = lu[18 + rv[0]] * ((lane == 12) ? 1.0 : 0.0);
-        z[0] = fma(-l, xk, z[0]); }
-    }
-    { const double xk = readlane_d(z[0], 15);
+        z[0] = fma(-l, x10, z[0]); }
       { const double l = lu[15 + rv[0]] * ((lane == 12) ? 1.0 : 0.0);
-        z[0] = fma(-l, xk, z[0]); }
-    }
-    { const double xk = readlane_d(z[0], 12);
-      { const double l = lu[12 + rv[0]] * ((lane == 6) ? 1.0 : 0.0);
-        z[0] = fma(-l, xk, z[0]); }
-    }
-    { const double xk = readlane_d(z[0], 9);
+        z[0] = fma(-l, x9, z[0]); }
       { const double l = lu[9 + rv[0]] * ((lane == 6) ? 1.0 : 0.0);
-        z[0] = fma(-l, xk, z[0]); }
-    }
-    { const double xk = readlane_d(z[0], 6);
-      { const double l = lu[6 + rv[0]] * ((lane == 0) ? 1.0 : 0.0);
-        z[0] = fma(-l, xk, z[0]); }
-    }
-    { const double xk = readlane_d(z[0], 3);
+        z[0] = fma(-l, x7, z[0]); }
       { const double l = lu[3 + rv[0]] * ((lane == 0) ? 1.0 : 0.0);
-        z[0] = fma(-l, xk, z[0]); }
+        z[0] = fma(-l, x5, z[0]); }
     }
-    { const double xk = readlane_d(z[0], 0);
+    {  // level 1: pivots 8
+      const double x8 = readlane_d(z[0], 12);
+      { const double l = lu[12 + rv[0]] * ((lane == 6) ? 1.0 : 0.0);
+        z[0] = fma(-l, x8, z[0]); }
+    }
+    {  // level 2: pivots 6
+      const double x6 = readlane_d(z[0], 6);
+      { const double l = lu[6 + rv[0]] * ((lane == 0) ? 1.0 : 0.0);
+        z[0] = fma(-l, x6, z[0]); }
+    }
+    {  // level 3: pivots 4
+      const double x4 = readlane_d(z[0], 0);
       { const double l = lu[0 + rv[0]] * (((0x0000000801000000ull >> lane) & 1ull) ? 1.0 : 0.0);
-        z[0] = fma(-l, xk, z[0]); }
+        z[0] = fma(-l, x4, z[0]); }
     }
   }
 };

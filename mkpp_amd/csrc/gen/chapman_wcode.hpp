@@ -121,33 +121,37 @@ struct Code {
     x[0] = wv[0] * rdiag[0];
   }
 
-  // U^T z = b, column oriented over U's rows: z_k = b_k / U(k,k) once column k
-  // is reached, then every j > k in row k of U subtracts U(k,j) z_k.  The lane
-  // holding row j (rv) reads U(k,j) at R[k] + j.
+  // U^T z = b, column oriented over U's rows in level order: z_k = b_k / U(k,k)
+  // once column k is reached, then every j > k in row k of U subtracts
+  // U(k,j) z_k.  The lane holding row j (rv) reads U(k,j) at R[k] + j.
   static __device__ __forceinline__ void tforward(const double* __restrict__ lu, const int (&rv)[1], double (&b)[1], const double (&rdiag)[1], double (&z)[1], int lane) {
     // b_k is final once column k is reached, so z = b * (1 / U(k,k)) is formed
     // for every row at the end.
-    { const double zk = readlane_d(b[0] * rdiag[0], 0);
+    {  // level 0: pivots 0
+      const double x0 = readlane_d(b[0] * rdiag[0], 0);
       { const double u = lu[0 + rv[0]] * (((0x0000000000000048ull >> lane) & 1ull) ? 1.0 : 0.0);
-        b[0] = fma(-u, zk, b[0]); }
+        b[0] = fma(-u, x0, b[0]); }
     }
-    { const double zk = readlane_d(b[0] * rdiag[0], 3);
+    {  // level 1: pivots 1
+      const double x1 = readlane_d(b[0] * rdiag[0], 3);
       { const double u = lu[3 + rv[0]] * ((lane == 6) ? 1.0 : 0.0);
-        b[0] = fma(-u, zk, b[0]); }
+        b[0] = fma(-u, x1, b[0]); }
     }
     z[0] = b[0] * rdiag[0];
   }
 
-  // L^T x = z in place (unit diagonal): x_k = z_k when column k is reached,
-  // descending; every j < k in row k of L subtracts L(k,j) x_k.
+  // L^T x = z in place (unit diagonal), in descending level order: x_k = z_k
+  // when column k is reached; every j < k in row k of L subtracts L(k,j) x_k.
   static __device__ __forceinline__ void tbackward(const double* __restrict__ lu, const int (&rv)[1], double (&z)[1], int lane) {
-    { const double xk = readlane_d(z[0], 6);
+    {  // level 0: pivots 2
+      const double x2 = readlane_d(z[0], 6);
       { const double l = lu[6 + rv[0]] * (((0x0000000000000009ull >> lane) & 1ull) ? 1.0 : 0.0);
-        z[0] = fma(-l, xk, z[0]); }
+        z[0] = fma(-l, x2, z[0]); }
     }
-    { const double xk = readlane_d(z[0], 3);
+    {  // level 1: pivots 1
+      const double x1 = readlane_d(z[0], 3);
       { const double l = lu[3 + rv[0]] * ((lane == 0) ? 1.0 : 0.0);
-        z[0] = fma(-l, xk, z[0]); }
+        z[0] = fma(-l, x1, z[0]); }
     }
   }
 };

@@ -979,6 +979,25 @@ def emit_code(t: dict, ct: dict) -> str:
             groups.append((f"level {lv}: pivots {', '.join(str(k) for k in ks)}", head, tms))
         emit_terms(groups, name, bases)
 
+    def emit_levels_factor(level, order, masks_fn, value_fn, name, addr_fn, acc):
+        """Level order as emit_levels, each factor entry scaled by a computed 0/1
+        lane factor that the scheduler places freely (the adjoint's transposed
+        solves: there the chunked loads and SGPR-pair masks of emit_terms timed
+        slower, 86.5 against 89.8 ms for the 65536-cell sweep)."""
+        piv = [k for k in order if any(masks_fn(k))]
+        for lv in sorted(set(level[k] for k in piv)):
+            ks = [k for k in piv if level[k] == lv]
+            w(f"    {{  // level {lv}: pivots {', '.join(str(k) for k in ks)}")
+            for k in ks:
+                w(f"      const double x{k} = readlane_d({value_fn(k)}, {slot[k] % 64});")
+            for k in ks:
+                masks = masks_fn(k)
+                for b in range(nb):
+                    if masks[b]:
+                        w(f"      {{ const double {name} = {addr_fn(k, b)} * ({_lane_test(masks[b])} ? 1.0 : 0.0);")
+                        w(f"        {acc}[{b}] = fma(-{name}, x{k}, {acc}[{b}]); }}")
+            w("    }")
+
     w("  // Solves: every lane loads its entry (in bounds for any row and column) a")
     w("  // chunk of terms ahead, and a select on the column's constant lane mask (an")
     w("  // SGPR pair) zeroes it outside the column's rows, off the substitution chain.")
@@ -1011,43 +1030,34 @@ def emit_code(t: dict, ct: dict) -> str:
     w("  }")
     w("")
     # ---- transposed solves (adjoint): W^T = U^T L^T ----
-    w("  // U^T z = b, column oriented over U's rows: z_k = b_k / U(k,k) once column k")
-    w("  // is reached, then every j > k in row k of U subtracts U(k,j) z_k.  The lane")
-    w("  // holding row j (rv) reads U(k,j) at R[k] + j.")
+    w("  // U^T z = b, column oriented over U's rows in level order: z_k = b_k / U(k,k)")
+    w("  // once column k is reached, then every j > k in row k of U subtracts")
+    w("  // U(k,j) z_k.  The lane holding row j (rv) reads U(k,j) at R[k] + j.")
     w("  static __device__ __forceinline__ void tforward(const double* __restrict__ lu, const int (&rv)[" + str(nb) + "], double (&b)[" + str(nb) + "], const double (&rdiag)[" + str(nb) + "], double (&z)[" + str(nb) + "], int lane) {")
     w("    // b_k is final once column k is reached, so z = b * (1 / U(k,k)) is formed")
     w("    // for every row at the end.")
-    # One pivot after another, each factor entry scaled by a computed 0/1
-    # lane factor.  (Level order with chunked loads and SGPR-pair masks, as in
-    # the forward solves, timed slower in the adjoint sweep: 87.4 -> 89.8 ms.)
+    # level schedule: z_j is final once every k < j with (k, j) in U is done
+    levt = [0] * nv
     for k in range(nv):
-        masks = masks_of([j for (kk, j) in Uset if kk == k and j > k])
-        if not any(masks):
-            continue
-        bk, lk = slot[k] // 64, slot[k] % 64
-        w(f"    {{ const double zk = readlane_d(b[{bk}] * rdiag[{bk}], {lk});")
-        for bb in range(nb):
-            if masks[bb]:
-                w(f"      {{ const double u = lu[{R[k]} + rv[{bb}]] * ({_lane_test(masks[bb])} ? 1.0 : 0.0);")
-                w(f"        b[{bb}] = fma(-u, zk, b[{bb}]); }}")
-        w("    }")
+        for j in sorted(j for (kk, j) in Uset if kk == k and j > k):
+            levt[j] = max(levt[j], levt[k] + 1)
+    emit_levels_factor(levt, range(nv), lambda k: masks_of([j for (kk, j) in Uset if kk == k and j > k]),
+                       lambda k: f"b[{slot[k] // 64}] * rdiag[{slot[k] // 64}]", "u",
+                       lambda k, bb: f"lu[{R[k]} + rv[{bb}]]", "b")
     for bb in range(nb):
         w(f"    z[{bb}] = b[{bb}] * rdiag[{bb}];")
     w("  }")
     w("")
-    w("  // L^T x = z in place (unit diagonal): x_k = z_k when column k is reached,")
-    w("  // descending; every j < k in row k of L subtracts L(k,j) x_k.")
+    w("  // L^T x = z in place (unit diagonal), in descending level order: x_k = z_k")
+    w("  // when column k is reached; every j < k in row k of L subtracts L(k,j) x_k.")
     w("  static __device__ __forceinline__ void tbackward(const double* __restrict__ lu, const int (&rv)[" + str(nb) + "], double (&z)[" + str(nb) + "], int lane) {")
+    # level schedule, descending: x_j is final once every k > j with (k, j) in L is done
+    levl = [0] * nv
     for k in reversed(range(nv)):
-        masks = masks_of([j for (kk, j) in Lset if kk == k])
-        if not any(masks):
-            continue
-        w(f"    {{ const double xk = readlane_d(z[{slot[k] // 64}], {slot[k] % 64});")
-        for bb in range(nb):
-            if masks[bb]:
-                w(f"      {{ const double l = lu[{R[k]} + rv[{bb}]] * ({_lane_test(masks[bb])} ? 1.0 : 0.0);")
-                w(f"        z[{bb}] = fma(-l, xk, z[{bb}]); }}")
-        w("    }")
+        for j in [j for (kk, j) in Lset if kk == k]:
+            levl[j] = max(levl[j], levl[k] + 1)
+    emit_levels_factor(levl, list(reversed(range(nv))), lambda k: masks_of([j for (kk, j) in Lset if kk == k]),
+                       lambda k: f"z[{slot[k] // 64}]", "l", lambda k, bb: f"lu[{R[k]} + rv[{bb}]]", "z")
     w("  }")
     w("};")
     w("")
