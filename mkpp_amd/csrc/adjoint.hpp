@@ -46,13 +46,30 @@ constexpr bool adjoint_wh_in_lu() {
   return adjoint_wh_doubles<W, Tab>() <= W::SPAN + 1;
 }
 
-// LDS doubles of the adjoint: the forward layout plus rbar and K of every
-// stage, the u vector of a J^T pass and the fixed-species adjoint (and the
-// second-derivative area where the LU block cannot hold it).
+// The scatter's chunk partials (SZERO + 1 doubles) use the F chunk partials'
+// area and the u vector behind it, both dead by then; `extra` pads the two
+// where they are shorter.
+template <class W>
+constexpr int adjoint_cp_extra() {
+  constexpr int need = WaveTables<W>::Code::SZERO + 1 - 2 * (W::NV + 1);
+  return need > 0 ? need : 0;
+}
+// The group dots of a J^T pass sit in the partials' area, read before any
+// partial is written, where that area holds them.
+template <class W>
+constexpr bool adjoint_wj_in_scr() {
+  return W::NG + 1 <= W::NSCR;
+}
+
+// LDS doubles of the adjoint: the forward layout, the scatter padding, the u
+// vector of a J^T pass (in the second-derivative pass: Fbar of stage 0), one
+// by-row vector per stage that holds Kbar_s until rbar_s replaces it, K of
+// every stage by row, the fixed-species adjoint, and the group dots and
+// second-derivative areas where the partials' and LU blocks cannot hold them.
 template <class W, class Tab>
 constexpr int adjoint_lds_doubles() {
-  return W::LDS_DOUBLES + (2 * Tab::S + 1) * (W::NV + 1) + Tab::S * W::NY + (W::NV + 1) + (W::NF + 1) + W::NY +
-         (W::NG + 1) + WaveTables<W>::Code::SEXTRA + (adjoint_wh_in_lu<W, Tab>() ? 0 : adjoint_wh_doubles<W, Tab>());
+  return W::LDS_DOUBLES + adjoint_cp_extra<W>() + (W::NV + 1) + 2 * Tab::S * (W::NV + 1) + (W::NF + 1) +
+         (adjoint_wj_in_scr<W>() ? 0 : W::NG + 1) + (adjoint_wh_in_lu<W, Tab>() ? 0 : adjoint_wh_doubles<W, Tab>());
 }
 
 template <class W, class Tab>
@@ -64,16 +81,22 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
   double* lu = lds;
   double* ylds = lu + (W::SPAN + 1);
   double* scr = ylds + NY;
-  double* facc = scr + W::NSCR;
-  double* rbv = facc + (NV + 1);     // [S+1][NV+1] rbar_s by row; row S: Fbar of stage 0
-  double* kv = rbv + (S + 1) * (NV + 1);  // [S][NY] K_s by y slot (fixed species and ONE: 0)
-  double* kbv = kv + S * NY;         // [S][NV+1] Kbar_s by row
-  double* uv = kbv + S * (NV + 1);   // [NV+1]    u of a J^T pass by row
-  double* fbar = uv + (NV + 1);      // [NF]      adjoint of the fixed species
-  double* sacc = fbar + (NF + 1);    // [NY]      with wj (+ SEXTRA): scatter chunk partials
-  double* wj = sacc + NY;            // [NG+1]    nu_g . u of a J^T pass
-  // [S+1][NG+1] nu_g . rbv rows (second-derivative pass, after the LU is done)
-  double* wh = adjoint_wh_in_lu<W, Tab>() ? lu : wj + (NG + 1) + C_::SEXTRA;
+  double* facc = scr + W::NSCR;      // [NV+1]    F chunk partials (recomputed stages only)
+  double* uv = facc + (NV + 1) + adjoint_cp_extra<W>();  // [NV+1] u of a J^T pass by row
+  // [S][NV+1] by row: Kbar_s while the recursion accumulates it, then rbar_s
+  // (the transposed solve of stage s reads Kbar_s last).  With uv in front,
+  // uv and these rows are the S + 1 vectors of the second-derivative pass
+  // (uv holding Fbar of stage 0 there).
+  double* rbv = uv + (NV + 1);
+  double* kbv = rbv;
+  double* kv = rbv + S * (NV + 1);   // [S][NV+1] K_s by row, slot NV = 0.0
+  double* fbar = kv + S * (NV + 1);  // [NF]      adjoint of the fixed species
+  // scatter chunk partials over facc, the padding and uv (all dead by then)
+  double* sacc = facc;
+  // [NG+1] nu_g . u of a J^T pass
+  double* wj = adjoint_wj_in_scr<W>() ? scr : fbar + (NF + 1);
+  // [S+1][NG+1] nu_g . (uv, rbv rows) (second-derivative pass, after the LU is done)
+  double* wh = adjoint_wh_in_lu<W, Tab>() ? lu : fbar + (NF + 1) + (adjoint_wj_in_scr<W>() ? 0 : NG + 1);
 
   // lane slot (b, lane) holds row ROWOF[b * 64 + lane] (255: none)
   int rb[NB], rowv[NB];
@@ -127,10 +150,10 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
 #pragma unroll
   for (int b = 0; b < NB; ++b) lam[b] = valid[b] ? g.lambda[cell + a.stride * (long)T_::VARPHYS[rowv[b]]] : 0.0;
   if (lane < NF) fbar[lane] = g.lambda[cell + a.stride * (long)T_::FIXPHYS[lane]];
-  for (int i = lane; i < S * NY; i += 64) kv[i] = 0.0;
-  // slot NV of the by-row vectors stays 0.0: the padding terms of the gather read it
-  if (lane <= S) rbv[lane * (NV + 1) + NV] = 0.0;
-  if (lane == 0) uv[NV] = 0.0;
+  for (int i = lane; i < S * (NV + 1); i += 64) kv[i] = 0.0;
+  // slot NV of the by-row vectors stays 0.0: the padding terms of the gather
+  // read it (uv's is set again before each use: the scatter may overwrite it)
+  if (lane < S) rbv[lane * (NV + 1) + NV] = 0.0;
 
   auto store_y = [&](const double (&y)[NB]) {
 #pragma unroll
@@ -233,8 +256,8 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
   // scr[partial slot] summed per species slot: rows into out[], fixed species
   // into fbar.  Two levels (CS tables): chunks of up to TS partial slots, any
   // species, summed by lane into cp[chunk]; then each species' lane adds its
-  // chunks in order.  cp spans the species-sum and group-dot areas, whose
-  // contents (the dots) are consumed before the scatter.
+  // chunks in order.  cp spans the F chunk partials, the padding and the u
+  // vector, whose contents are consumed before the scatter.
   auto scatter = [&](double (&out)[NB]) {
     double* cp = sacc;
     if (lane == 0) scr[W::NPART] = 0.0;
@@ -286,12 +309,18 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
   };
 
   auto jt_pass = [&](double (&out)[NB]) {
+    if (lane == 0) uv[NV] = 0.0;
     group_dots(std::integral_constant<int, 1>{}, uv, wj);
+    // every dot is read before the first partial is written (wj may be the
+    // partials' area; one wavefront's LDS accesses complete in order)
+    double wg[NGP];
+#pragma unroll
+    for (int p = 0; p < NGP; ++p) wg[p] = p * 64 + lane < NG ? wj[p * 64 + lane] : 0.0;
 #pragma unroll
     for (int p = 0; p < NGP; ++p) {
       const int gi = p * 64 + lane;
       if (gi < NG) {
-        const double w = wj[gi];
+        const double w = wg[p];
         double pw[3], pm1[3], pm2[3];
         monomial(gi, pw, pm1, pm2);
         kbar_add(p, w * ((pw[0] * pw[1]) * pw[2]));
@@ -313,9 +342,11 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
 
   // second-derivative terms at y_n: sum_s d/dy [rbar_s^T J K_s] and d/dk
   // Also carries the J^T pass of stage 0, which is evaluated at y_n too: its
-  // Fbar is row S of rbv, and its terms join the same partial slots.
+  // Fbar is in uv, the vector in front of the rbv rows, and its terms join
+  // the same partial slots.
   auto hess_pass = [&](double (&out)[NB]) {
-    group_dots(std::integral_constant<int, S + 1>{}, rbv, wh);
+    if (lane == 0) uv[NV] = 0.0;
+    group_dots(std::integral_constant<int, S + 1>{}, uv, wh);
 #pragma unroll 1
     for (int p = 0; p < NGP; ++p) {
       const int gi = p * 64 + lane;
@@ -354,8 +385,8 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
         }
         double ws[S];
 #pragma unroll
-        for (int s = 0; s < S; ++s) ws[s] = wh[s * (NG + 1) + gi];
-        const double w0 = wh[S * (NG + 1) + gi];
+        for (int s = 0; s < S; ++s) ws[s] = wh[(s + 1) * (NG + 1) + gi];
+        const double w0 = wh[gi];
         double val[3] = {w0 * d1[0], w0 * d1[1], w0 * d1[2]};
         double kb = w0 * ((pw[0] * pw[1]) * pw[2]);
 #pragma unroll
@@ -363,7 +394,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
           const double w = ws[s];
           double kvs[3];
 #pragma unroll
-          for (int e = 0; e < 3; ++e) kvs[e] = kv[s * NY + sk[e]];
+          for (int e = 0; e < 3; ++e) kvs[e] = kv[s * (NV + 1) + (sk[e] < NV ? sk[e] : NV)];
           double dk = 0.0;
 #pragma unroll
           for (int e = 0; e < 3; ++e) {
@@ -449,12 +480,12 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
           for (int b = 0; b < NB; ++b) K[j][b] = xs[b];
         }
     }
-    // K_s by y slot for the second-derivative pass
+    // K_s by row for the recursion and the second-derivative pass
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
       for (int b = 0; b < NB; ++b)
-        if (valid[b]) kv[s * NY + rowv[b]] = K[s][b];
+        if (valid[b]) kv[s * (NV + 1) + rowv[b]] = K[s][b];
 
     // ---- backward stage recursion (K and Kbar live in LDS, by row) ----
     double yb[NB], pend[NB];
@@ -503,7 +534,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
           if (c == 0.0) continue;
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
-            const double kj = valid[b] ? kv[j * NY + rowv[b]] : 0.0;
+            const double kj = valid[b] ? kv[j * (NV + 1) + rowv[b]] : 0.0;
             Y[b] = (c == 1.0) ? Y[b] + kj : Y[b] + c * kj;
           }
         }
@@ -512,7 +543,7 @@ __device__ __forceinline__ void wave_adjoint_cell(const DeviceArgs& a, const Adj
           // stage 0 (Y = y_n): its J^T pass is merged into the second-derivative pass
 #pragma unroll
           for (int b = 0; b < NB; ++b)
-            if (valid[b]) rbv[S * (NV + 1) + rowv[b]] = pend[b];
+            if (valid[b]) uv[rowv[b]] = pend[b];
           continue;
         }
 #pragma unroll

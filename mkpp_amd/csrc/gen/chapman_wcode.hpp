@@ -23,11 +23,9 @@ struct Code {
   static constexpr int FMAXC[1] = {1};
   static constexpr int TS = 8, NSP = 1;  // adjoint scatter chunks
   // scatter chunk partials cp[0, 64 NSP), cp[SZERO] = 0.0; rows read SMAXC[bank] of
-  // their SMAXCP-wide list, fixed species SFMAXC of SFMAXCP; SEXTRA: LDS the
-  // partials need beyond the adjoint's species-sum and group-dot areas
+  // their SMAXCP-wide list, fixed species SFMAXC of SFMAXCP
   static constexpr int SZERO = 64, SMAXCP = 4, SFMAXC = 1, SFMAXCP = 1;
   static constexpr int SMAXC[1] = {1};
-  static constexpr int SEXTRA = 55;
   static constexpr int TG = 4, NGAT = 1;  // adjoint gather chunks
 
   // Right-looking factorization: pivot k divides column k of every row in

@@ -679,11 +679,10 @@ def build_s_tables(t: dict, ts: int = F_CHUNK_TERMS) -> dict:
     for f in range(max(nf, 1)):
         ps = parts[nv + f] if f < nf else []
         sfix += ps + [szero] * (sfmaxcp - len(ps))
-    # the partials live in the adjoint's species-sum and group-dot areas
-    # (NY + NG + 1 doubles, both free during the scatter) and beyond if needed
-    sextra = max(0, szero + 1 - (t["ny"] + t["ng"] + 1))
+    # the partials' SZERO + 1 doubles live in the adjoint's F chunk partials
+    # and u vector areas, padded there as needed (adjoint_cp_extra)
     return {"ts": ts, "nsp": npass, "sterm": words, "spart": spart, "sfix": sfix, "szero": szero,
-            "smaxc": smaxc, "smaxcp": smaxcp, "sfmaxc": sfmaxc, "sfmaxcp": sfmaxcp, "sextra": sextra}
+            "smaxc": smaxc, "smaxcp": smaxcp, "sfmaxc": sfmaxc, "sfmaxcp": sfmaxcp}
 
 
 def build_code_tables(t: dict, tf: int = F_CHUNK_TERMS, tw: int = 2) -> dict:
@@ -807,11 +806,9 @@ def emit_code(t: dict, ct: dict) -> str:
     w(f"  static constexpr int FMAXC[{nb}] = {{{', '.join(str(m) for m in ct['fmaxc'])}}};")
     w(f"  static constexpr int TS = {ct['ts']}, NSP = {ct['nsp']};  // adjoint scatter chunks")
     w(f"  // scatter chunk partials cp[0, 64 NSP), cp[SZERO] = 0.0; rows read SMAXC[bank] of")
-    w(f"  // their SMAXCP-wide list, fixed species SFMAXC of SFMAXCP; SEXTRA: LDS the")
-    w(f"  // partials need beyond the adjoint's species-sum and group-dot areas")
+    w(f"  // their SMAXCP-wide list, fixed species SFMAXC of SFMAXCP")
     w(f"  static constexpr int SZERO = {ct['szero']}, SMAXCP = {ct['smaxcp']}, SFMAXC = {ct['sfmaxc']}, SFMAXCP = {ct['sfmaxcp']};")
     w(f"  static constexpr int SMAXC[{nb}] = {{{', '.join(str(m) for m in ct['smaxc'])}}};")
-    w(f"  static constexpr int SEXTRA = {ct['sextra']};")
     w(f"  static constexpr int TG = {ct['tg']}, NGAT = {ct['ngat']};  // adjoint gather chunks")
     w("")
     # ---- factorization ----
