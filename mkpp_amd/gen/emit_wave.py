@@ -508,8 +508,15 @@ SOLVE_PREFETCH = 6
 # ones those pivots update most, per mechanism.  Timed on the bench cells:
 # saprc99 Rodas3 120.7 -> 114.4 ms with 12 (8: 115.0; 13 or more put the
 # kernel past its SGPR budget and spill masks to VGPR lanes: 126 ms);
-# saprcnov 449 -> 432 ms with 12.
-LDS_PHASE_REGISTER_COLUMNS = {"saprc99": 12, "saprcnov": 12}
+# saprcnov 449 -> 432 ms with 12; ts1 (175 LDS-phase pivots, one wavefront
+# per SIMD, AGPRs to spare) 635 -> 502 ms with 24 (512 cells, 5000 Ros3
+# attempts each).
+LDS_PHASE_REGISTER_COLUMNS = {"saprc99": 12, "saprcnov": 12, "ts1": 24}
+# Pivot columns of the LDS phase held in registers until their own pivot has
+# divided them, the most updated ones: ts1 502 -> 466 ms with 16 (48 spill
+# to scratch); saprc99 gains 1.2 % and saprcnov 2.5 % at most, inside the
+# spread between runs, and keep their layout.
+LDS_PHASE_PIVOT_COLUMNS = {"ts1": 16}
 
 
 def _lane_test(mask: int, var: str = "lane") -> str:
@@ -835,7 +842,12 @@ def emit_code(t: dict, ct: dict) -> str:
         for j in t["ucol"][t["uofs"][k]:t["uofs"][k + 1]]:
             upd[j] = upd.get(j, 0) + nrow
     nrc = LDS_PHASE_REGISTER_COLUMNS.get(name, 0)
+    npc = LDS_PHASE_PIVOT_COLUMNS.get(name, 0)
     rc = sorted(sorted((j for j in upd if j >= k0), key=lambda j: (-upd[j], j))[:nrc])
+    # pivot columns of the phase itself, likewise: held from the start of the
+    # factorization until their own pivot has divided them, then stored
+    pc = sorted(sorted((j for j in upd if j < k0), key=lambda j: (-upd[j], j))[:npc])
+    rc = pc + rc
     rcb = {j: [b for b in range(nb) if masks_of([r for r in range(nv) if (r, j) in pattern])[b]] for j in rc}
     if rc:
         w(f"    // columns {', '.join(str(j) for j in rc)} in registers through the pivots below")
@@ -850,7 +862,10 @@ def emit_code(t: dict, ct: dict) -> str:
             continue
         cols = t["ucol"][t["uofs"][k]:t["uofs"][k + 1]]
         w(f"    {{  // pivot {k}: {len(rows)} rows x {len(cols)} columns")
-        w(f"      const double d = pv[{R[k] + k}];")
+        if k in rc:
+            w(f"      const double d = readlane_d(e{k}_{slot[k] // 64}, {slot[k] % 64});")
+        else:
+            w(f"      const double d = pv[{R[k] + k}];")
         for j in cols:
             if j in rc:
                 w(f"      const double u{j} = readlane_d(e{j}_{slot[k] // 64}, {slot[k] % 64});")
@@ -862,17 +877,29 @@ def emit_code(t: dict, ct: dict) -> str:
             if not m:
                 continue
             w(f"      if ({ms.test(m)}) {{")
-            w(f"        double* r = lu + rb[{b}];")
-            w(f"        const double l = r[{k}] / d;")
-            w(f"        r[{k}] = l;")
+            if any(j not in rc for j in cols) or k not in rc:
+                w(f"        double* r = lu + rb[{b}];")
+            if k in rc:
+                w(f"        const double l = e{k}_{b} / d;")
+                w(f"        e{k}_{b} = l;")
+            else:
+                w(f"        const double l = r[{k}] / d;")
+                w(f"        r[{k}] = l;")
             for j in cols:
                 if j in rc:
                     w(f"        e{j}_{b} = fma(-l, u{j}, e{j}_{b});")
                 else:
                     w(f"        r[{j}] = fma(-l, u{j}, r[{j}]);")
             w("      }")
+        if k in rc:
+            # column k is final: multipliers below the diagonal, U above it
+            for b in rcb[k]:
+                m = masks_of([r for r in range(nv) if (r, k) in pattern])[b]
+                w(f"      if ({ms.test(m)}) lu[rb[{b}] + {k}] = e{k}_{b};")
         w("    }")
     for j in rc:
+        if j < k0:
+            continue
         # rows above the trailing block are final; the block's rows go on in c<j>
         for b in rcb[j]:
             m = masks_of([r for r in range(nv) if (r, j) in pattern and r < k0])[b]
