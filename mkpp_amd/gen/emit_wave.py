@@ -16,6 +16,7 @@ through the scalar cache and per-lane tables through L1.
 
 from __future__ import annotations
 
+import json
 import os
 
 from .emit_hip import VarMap
@@ -143,7 +144,7 @@ def build_tables(mech: Mechanism, p: dict) -> dict:
         if i < k:
             part[i] += 1
     # the trailing block factorized in registers (rows >= k0) must sit in bank 0
-    k0 = max(0, nv - REGISTER_BLOCK)
+    k0 = max(0, nv - knobs(mech.name)["register_block"])
     if nb == 1:
         banks = [list(range(nv))]
     else:
@@ -496,27 +497,41 @@ def write(mech: Mechanism, p: dict, outdir: str) -> list:
 # fixed-trip-count term tables for F and W (the compiler can batch-issue every
 # table load and LDS access because no loop bound depends on data).
 # ---------------------------------------------------------------------------
-# Trailing rows of the LU factorized in registers (lanes = rows, one register per
-# column); 35 columns of doubles are 70 VGPRs.
-REGISTER_BLOCK = 35
-# Pivot-row broadcasts of the register block issued together (see factor).
-BROADCAST_GROUP = 4
-# Solve factor entries are loaded in chunks of this many terms, one chunk
-# ahead of the updates (6 timed fastest of 4-12 on the saprc99 Rodas3 cells).
-SOLVE_PREFETCH = 6
-# Trailing-block columns held in registers through the LDS-phase pivots, the
-# ones those pivots update most, per mechanism.  Timed on the bench cells:
-# saprc99 Rodas3 120.7 -> 114.4 ms with 12 (8: 115.0; 13 or more put the
-# kernel past its SGPR budget and spill masks to VGPR lanes: 126 ms);
-# saprcnov 449 -> 432 ms with 12; ts1 (175 LDS-phase pivots, one wavefront
-# per SIMD, AGPRs to spare) 635 -> 502 ms with 24 (512 cells, 5000 Ros3
-# attempts each).
-LDS_PHASE_REGISTER_COLUMNS = {"saprc99": 12, "saprcnov": 12, "ts1": 24}
-# Pivot columns of the LDS phase held in registers until their own pivot has
-# divided them, the most updated ones: ts1 502 -> 466 ms with 16 (48 spill
-# to scratch); saprc99 gains 1.2 % and saprcnov 2.5 % at most, inside the
-# spread between runs, and keep their layout.
-LDS_PHASE_PIVOT_COLUMNS = {"ts1": 16}
+# Code-shape knobs of the wave engine.  None of them changes a result (every
+# setting computes the same operations in the same per-entry order); they
+# trade registers, scheduling freedom and LDS round trips.  Defaults here,
+# per-mechanism values chosen by the autotuner (`python -m mkpp_amd.tune`) in
+# mkpp_amd/tuning/wave_knobs.json.
+KNOB_DEFAULTS = {
+    # trailing rows of the LU factorized in registers (lanes = rows, one
+    # register per column; 35 columns of doubles are 70 VGPRs)
+    "register_block": 35,
+    # pivot-row broadcasts of the register block issued together (see factor)
+    "broadcast_group": 4,
+    # solve factor entries loaded in chunks of this many terms, one chunk
+    # ahead of the updates
+    "solve_prefetch": 6,
+    # trailing-block columns held in registers through the LDS-phase pivots,
+    # the ones those pivots update most
+    "register_columns": 0,
+    # LDS-phase pivot columns held in registers until their own pivot has
+    # divided them, the most updated ones
+    "pivot_columns": 0,
+}
+KNOBS_FILE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tuning", "wave_knobs.json")
+# settings that replace the file's (the autotuner's candidates)
+KNOB_OVERRIDES: dict = {}
+
+
+def knobs(name: str) -> dict:
+    """The code-shape knobs of one mechanism: defaults, then the tuned file,
+    then KNOB_OVERRIDES."""
+    k = dict(KNOB_DEFAULTS)
+    if os.path.exists(KNOBS_FILE):
+        tuned = json.load(open(KNOBS_FILE)).get(name, {})
+        k.update({key: tuned[key] for key in KNOB_DEFAULTS if key in tuned})
+    k.update({key: v for key, v in KNOB_OVERRIDES.get(name, {}).items() if key in KNOB_DEFAULTS})
+    return k
 
 
 def _lane_test(mask: int, var: str = "lane") -> str:
@@ -769,6 +784,7 @@ def build_code_tables(t: dict, tf: int = F_CHUNK_TERMS, tw: int = 2) -> dict:
 
 def emit_code(t: dict, ct: dict) -> str:
     name = t["name"]
+    kn = knobs(name)
     nv, nb = t["nv"], t["nb"]
     R = t["R"]
     Lset = set()
@@ -841,8 +857,8 @@ def emit_code(t: dict, ct: dict) -> str:
         nrow = t["lofs"][k + 1] - t["lofs"][k]
         for j in t["ucol"][t["uofs"][k]:t["uofs"][k + 1]]:
             upd[j] = upd.get(j, 0) + nrow
-    nrc = LDS_PHASE_REGISTER_COLUMNS.get(name, 0)
-    npc = LDS_PHASE_PIVOT_COLUMNS.get(name, 0)
+    nrc = kn["register_columns"]
+    npc = kn["pivot_columns"]
     rc = sorted(sorted((j for j in upd if j >= k0), key=lambda j: (-upd[j], j))[:nrc])
     # pivot columns of the phase itself, likewise: held from the start of the
     # factorization until their own pivot has divided them, then stored
@@ -928,8 +944,8 @@ def emit_code(t: dict, ct: dict) -> str:
             w(f"      c{k} = in_l ? l : c{k};")
             # broadcasts in groups: a group's readlanes issue together, ahead of
             # their FMAs, instead of each waiting out its SGPR hazard
-            for g0 in range(0, len(cols), BROADCAST_GROUP):
-                grp = cols[g0:g0 + BROADCAST_GROUP]
+            for g0 in range(0, len(cols), kn["broadcast_group"]):
+                grp = cols[g0:g0 + kn["broadcast_group"]]
                 for j in grp:
                     w(f"      const double u{j} = readlane_d(c{j}, {slot[k]});")
                 for j in grp:
@@ -954,7 +970,7 @@ def emit_code(t: dict, ct: dict) -> str:
         terms (addr, mask, bank, x)), in order.  Every term's factor entry is
         loaded a chunk ahead (`pf`) and kept in the lanes of its mask."""
         terms = [tm for (_, _, tms) in groups for tm in tms]
-        nt, d = len(terms), SOLVE_PREFETCH
+        nt, d = len(terms), kn["solve_prefetch"]
         sites = _MaskSites()
         w("    (void)lane;")
         if nt == 0:
