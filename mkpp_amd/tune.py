@@ -51,10 +51,10 @@ BUILD = os.path.join(ROOT, "build", "tune")
 
 # values tried per knob (the current setting is always tried as well)
 GRID = {
-    "register_columns": [0, 8, 12, 16, 24],
-    "pivot_columns": [0, 8, 16],
+    "register_columns": [0, 8, 12, 16, 24, 32],
+    "pivot_columns": [0, 8, 16, 24, 32, 48, 64],
     "solve_prefetch": [4, 6, 8],
-    "register_block": [33, 35, 37],
+    "register_block": [33, 35, 37, 39, 41],
     "broadcast_group": [2, 4, 8],
 }
 TABLEAUX = {"ros2": "TabRos2", "ros3": "TabRos3", "ros4": "TabRos4", "rodas3": "TabRodas3", "rodas4": "TabRodas4"}
