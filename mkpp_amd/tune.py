@@ -100,9 +100,14 @@ def compile_candidate(mech: str, k: dict, method: str) -> str:
 
 
 def workload(mech: str, cells: int) -> dict:
+    """The cells a mechanism is tuned on: its bench workload."""
     if mech == "ts1":
         return workloads.ts1_scenario(cells)
-    return getattr(workloads, f"{mech}_urban")(cells)
+    make = getattr(workloads, f"{mech}_urban", None)
+    if make is None:
+        raise ValueError(f"no tuning workload for {mech!r}: add mkpp_amd.workloads.{mech}_urban(cells) "
+                         "returning the same fields as saprc99_urban")
+    return make(cells)
 
 
 class Runner:

@@ -82,3 +82,8 @@ def test_concurrent_candidates_get_their_own_code(tmp_path):
         # the comment lists the pivot columns and the trailing-block columns
         assert (len(m.group(1).split(",")) if m else 0) == k["register_columns"] + k["pivot_columns"], tune.key_of(k)
         assert f"loaded {k['solve_prefetch']} to" in code, tune.key_of(k)
+
+
+def test_mechanism_without_a_workload_is_refused():
+    with pytest.raises(ValueError, match="no tuning workload"):
+        tune.workload("chapman", 64)
